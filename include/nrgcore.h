@@ -1,0 +1,167 @@
+/*
+ * nrgcore.h - C ABI of the MI355X (gfx950) per-step NRG numerical core.
+ *
+ * Plain C: no HIP, torch or C++ types cross this boundary. Device buffers are
+ * raw `double*` pointers into caller-owned device memory (hipMalloc'ed or a torch
+ * tensor's data pointer); descriptor tables live in host memory; the stream is
+ * passed as `void*` and is cast to hipStream_t inside the library (NULL = the
+ * null stream). Every entry point returns 0 on success and a nonzero NRG_E_*
+ * code otherwise; nrg_last_error() then gives a message for the calling thread.
+ * No exception ever crosses the boundary.
+ *
+ * Layout conventions follow the reference's ublas matrices (row-major,
+ * c++/traits.hpp:41): a `dim x dim` Hamiltonian block is row-major with its
+ * upper triangle filled (asserted by is_matrix_upper, c++/diag.hpp:265), and
+ * eigenvectors come back as ROWS of a row-major `dim x dim` matrix
+ * (c++/diag.hpp:41-48).
+ */
+#ifndef NRGCORE_H
+#define NRGCORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define NRG_OK            0
+#define NRG_E_ARG         1   /* invalid argument (sizes, NULL pointers, unknown mode) */
+#define NRG_E_HIP         2   /* a HIP runtime call failed */
+#define NRG_E_NOCONV      3   /* a block did not converge (see per-block info) */
+#define NRG_E_NOMEM       4   /* device workspace allocation failed */
+#define NRG_E_NODEV       5   /* no usable gfx950 device */
+
+/* eigensolver forms */
+#define NRG_SYEV_FP64     0   /* plain FP64: Householder + divide-and-conquer + back-transform */
+#define NRG_SYEV_MIXED    1   /* FP32 reduction/back-transform + FP64 Ogita-Aishima refinement */
+
+/* product precisions */
+#define NRG_GEMM_FP64     0   /* FP64 arithmetic */
+#define NRG_GEMM_OZAKI    1   /* split-precision BF16 slices, FP32 accumulation, FP64 recombination */
+
+typedef struct nrg_ctx nrg_ctx;
+
+/* Opaque context: owns the device workspace, descriptor staging and timers for
+ * one device. A context is not thread-safe; use one per host thread. */
+int nrg_ctx_create(nrg_ctx **ctx, int device);
+int nrg_ctx_destroy(nrg_ctx *ctx);
+const char *nrg_last_error(void);
+/* Version string of the library build ("nrgcore <git-describe> gfx950"). */
+const char *nrg_version(void);
+
+/*
+ * Batched real symmetric eigensolver: replaces the per-block loop
+ *   diagonalisations_OpenMP  (c++/core.hpp:90-110)  ->  diagonalise (c++/diag.hpp:261-292)
+ *   -> diagonalise_dsyev     (c++/diag.hpp:61-82, LAPACK dsyev with UPLO='L' on the row-major upper triangle).
+ * All `nblk` blocks of one NRG step are diagonalised in one call.
+ *
+ *   n[b]      (host)   order of block b (>= 1)
+ *   off_H[b]  (host)   element offset of block b inside H_arena
+ *   H_arena   (device) on entry: block b is a row-major n x n matrix whose upper
+ *                      triangle (j >= i) holds H; the strict lower triangle is ignored.
+ *                      On exit: row i of block b is the i-th eigenvector (unit norm),
+ *                      exactly the layout copy_vec() produces (c++/diag.hpp:41-48).
+ *   off_w[b]  (host)   element offset of block b inside w_arena
+ *   w_arena   (device) on exit: the n eigenvalues of block b in ascending order.
+ *   mode               NRG_SYEV_FP64 or NRG_SYEV_MIXED
+ *   info      (host)   per-block status, may be NULL: 0 = ok, >0 = block failed
+ *                      (non-finite input or no convergence; the reference throws
+ *                      std::runtime_error for dsyev INFO != 0, c++/diag.hpp:80).
+ *   stream             hipStream_t as void*; the call is synchronous on it.
+ * Returns NRG_E_NOCONV if any block failed.
+ */
+int nrg_syev_batched(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H_arena,
+                     const int64_t *off_w, double *w_arena, int mode, int *info, void *stream);
+
+/*
+ * One term of a grouped triple product, in the reference's row-major layout:
+ *   C[m x n] += f * opA[m x k] * O[k x l] * opB[l x n]
+ * with
+ *   opA = A        (transA = 0, A stored m x k)   or  A^T (transA = 1, A stored k x m)
+ *   opB = B^T      (transB = 0, B stored n x l)   or  B   (transB = 1, B stored l x n)
+ *   O == NULL      means the identity (then k must equal l).
+ * The three reference kernels map onto it as
+ *   product   M += f A B^T     (c++/numerics_ublas.hpp:108-116): transA=0, O=NULL, transB=0
+ *   transform M += f A O B^T   (c++/numerics_ublas.hpp:118-128): transA=0,          transB=0
+ *   rotate    M += f U^T O U   (c++/numerics_ublas.hpp:130-140): A=U, transA=1, B=U, transB=1
+ * Terms with the same C pointer are summed into that C (recalc_f / recalc_general
+ * table rows, c++/recalc.hpp:52-108; the combs children of cdmI, c++/dmnrg.hpp:54-80).
+ * Terms whose opA or opB has zero size contribute nothing (finite_size checks).
+ */
+typedef struct nrg_gemm3_term {
+  int64_t m, n, k, l;
+  double f;
+  const double *A; int64_t lda; int32_t transA;
+  const double *O; int64_t ldo;
+  const double *B; int64_t ldb; int32_t transB;
+  double *C; int64_t ldc;
+} nrg_gemm3_term;
+
+/* Grouped accumulation of all terms (host table, device matrices). Replaces the
+ * per-row product/transform loops of Symmetry::recalc_f / recalc_general
+ * (c++/recalc.hpp:52-108), Oprecalc::recalculate_operators (c++/oprecalc.hpp:70-87)
+ * and the rotate loop of calc_fulldensitymatrix_iterN (c++/dmnrg.hpp:164-197).
+ * All C matrices must be distinct from all A/O/B inputs. */
+int nrg_gemm3_grouped(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, int precision, void *stream);
+
+/*
+ * FDM spectral-weight accumulation for one NRG step: replaces the pair loop of
+ * BaseSpectrum::calc (c++/algo.hpp:55-67) driving Algo_FDM::calc
+ * (c++/algo_FDM.hpp:136-142 = Algo_FDMgt 84-114 + Algo_FDMls 30-60) and the
+ * binning ChainBinning::add -> Bins::add/add_std (c++/spectrum.hpp:19-28,
+ * c++/bins.hpp:103-132).
+ *
+ * For the subspace pair (Ii, Ij) the matrices op1, op2 hold <j|op|i>: they are
+ * dim_j x dim_i row-major (ld_op). States [0, K) of a subspace are "kept" (Krange)
+ * and [K, dim) "discarded" (Drange), c++/eigen.hpp:245-246. E_i, E_j are the
+ * abs_G energies (c++/eigen.hpp:47). rho_i / rho_j are the FDM density matrices
+ * (row-major, n_rho x n_rho); the products rho_j*op2 and op2*rho_i are formed as
+ * prod_fit does (c++/numerics.hpp:283-306).
+ */
+typedef struct nrg_fdm_pair {
+  int64_t dim_i, dim_j;
+  int64_t K_i, K_j;
+  const double *E_i, *E_j;
+  const double *op1, *op2; int64_t ld_op;
+  const double *rho_i; int64_t n_rho_i;
+  const double *rho_j; int64_t n_rho_j;
+  double factor;
+} nrg_fdm_pair;
+
+/* Standard logarithmic bin grid (Bins::loggrid_std, c++/bins.hpp:95-100):
+ * bin b sits at 10^(log10emin + b/bins_per_decade), b = 0..nbins-1. */
+typedef struct nrg_bin_grid {
+  double log10emin;
+  double bins_per_decade;
+  int64_t nbins;
+  double discard_immediately;   /* P.discard_immediately (c++/bins.hpp:105) */
+} nrg_bin_grid;
+
+/* Accumulates (+=) into four device arrays of nbins doubles each:
+ * the greater (gt) and lesser (ls) halves, for positive and negative energies.
+ * sign: +1 bosons, -1 fermions (gf_sign). T: temperature; wnf: stats.wnfactor[N]. */
+int nrg_fdm_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid,
+                double T, double wnf, int sign, double *gt_pos, double *gt_neg, double *ls_pos,
+                double *ls_neg, void *stream);
+
+/* Timing of the most recent nrg_syev_batched / nrg_gemm3_grouped / nrg_fdm_bin call on
+ * this context, in milliseconds, measured with HIP events on the call's stream.
+ * phase: 0 = whole call, 1.. = internal phases (see nrg_phase_name). */
+double nrg_last_time_ms(nrg_ctx *ctx, int phase);
+const char *nrg_phase_name(int phase);
+
+/* Optional per-kernel timing with HIP events around every launch on the call's stream (adds two
+ * event records per launch). nrg_kernel_timing(ctx, 1) enables it and clears the accumulators;
+ * nrg_kernel_time returns the summed milliseconds and launch count of one kernel (e.g. "k_panel")
+ * since then; nrg_kernel_names writes the comma-separated names seen so far into buf.
+ * No counterpart in the reference (its timers are the wall-clock sections of c++/time_mem.hpp). */
+int nrg_kernel_timing(nrg_ctx *ctx, int enable);
+int nrg_kernel_time(nrg_ctx *ctx, const char *name, double *total_ms, long *launches);
+int nrg_kernel_names(nrg_ctx *ctx, char *buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NRGCORE_H */

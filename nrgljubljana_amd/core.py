@@ -1,0 +1,179 @@
+"""ctypes binding of the gfx950 NRG core (libnrgcore.so, C ABI declared in include/nrgcore.h).
+
+Device memory comes from torch (ROCm build): tensors are plumbing here, every computation runs in
+the HIP kernels of libnrgcore.so. There is no CPU fallback: if the library is missing, or no gfx950
+device is present, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnrgcore.so")
+
+NRG_SYEV_FP64 = 0
+NRG_SYEV_MIXED = 1
+NRG_GEMM_FP64 = 0
+NRG_GEMM_OZAKI = 1
+
+EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
+            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
+            "nrg_kernel_time", "nrg_kernel_names"]
+
+
+class Gemm3Term(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("k", ctypes.c_int64), ("l", ctypes.c_int64),
+                ("f", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64), ("transA", ctypes.c_int32),
+                ("O", ctypes.c_void_p), ("ldo", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("transB", ctypes.c_int32),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64)]
+
+
+class FdmPair(ctypes.Structure):
+    _fields_ = [("dim_i", ctypes.c_int64), ("dim_j", ctypes.c_int64), ("K_i", ctypes.c_int64),
+                ("K_j", ctypes.c_int64), ("E_i", ctypes.c_void_p), ("E_j", ctypes.c_void_p),
+                ("op1", ctypes.c_void_p), ("op2", ctypes.c_void_p), ("ld_op", ctypes.c_int64),
+                ("rho_i", ctypes.c_void_p), ("n_rho_i", ctypes.c_int64),
+                ("rho_j", ctypes.c_void_p), ("n_rho_j", ctypes.c_int64), ("factor", ctypes.c_double)]
+
+
+class BinGrid(ctypes.Structure):
+    _fields_ = [("log10emin", ctypes.c_double), ("bins_per_decade", ctypes.c_double),
+                ("nbins", ctypes.c_int64), ("discard_immediately", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libnrgcore.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with nrgljubljana_amd.build.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
+    L.nrg_ctx_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+    L.nrg_ctx_create.restype = ctypes.c_int
+    L.nrg_ctx_destroy.argtypes = [vp]
+    L.nrg_ctx_destroy.restype = ctypes.c_int
+    L.nrg_last_error.restype = ctypes.c_char_p
+    L.nrg_version.restype = ctypes.c_char_p
+    L.nrg_syev_batched.argtypes = [vp, ctypes.c_int, i64p, i64p, vp, i64p, vp, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), vp]
+    L.nrg_syev_batched.restype = ctypes.c_int
+    L.nrg_gemm3_grouped.argtypes = [vp, ctypes.POINTER(Gemm3Term), ctypes.c_int, ctypes.c_int, vp]
+    L.nrg_gemm3_grouped.restype = ctypes.c_int
+    L.nrg_fdm_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
+                              ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, vp]
+    L.nrg_fdm_bin.restype = ctypes.c_int
+    L.nrg_last_time_ms.argtypes = [vp, ctypes.c_int]
+    L.nrg_last_time_ms.restype = ctypes.c_double
+    L.nrg_phase_name.argtypes = [ctypes.c_int]
+    L.nrg_phase_name.restype = ctypes.c_char_p
+    L.nrg_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.nrg_kernel_timing.restype = ctypes.c_int
+    L.nrg_kernel_time.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+    L.nrg_kernel_time.restype = ctypes.c_int
+    L.nrg_kernel_names.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.nrg_kernel_names.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+class NrgError(RuntimeError):
+    def __init__(self, code, where):
+        msg = lib().nrg_last_error().decode()
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+class Context:
+    """One device context (owns the device workspace of libnrgcore)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        r = lib().nrg_ctx_create(ctypes.byref(self._h), int(device))
+        if r != 0:
+            raise NrgError(r, "nrg_ctx_create")
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().nrg_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ eigensolver
+    def syev_batched(self, H, sizes, off_H, w, off_w, mode=NRG_SYEV_FP64, stream=None, check=True):
+        """H, w: flat float64 device tensors (arenas). Returns the per-block info array."""
+        sizes_a, p_n = _i64(sizes)
+        offH_a, p_oh = _i64(off_H)
+        offw_a, p_ow = _i64(off_w)
+        nblk = len(sizes_a)
+        info = (ctypes.c_int * max(nblk, 1))()
+        r = lib().nrg_syev_batched(self._h, nblk, p_n, p_oh, ctypes.c_void_p(H.data_ptr()), p_ow,
+                                   ctypes.c_void_p(w.data_ptr()), int(mode), info, _stream_ptr(stream))
+        info_np = np.array(info[:nblk], dtype=np.int32)
+        if r != 0 and check:
+            raise NrgError(r, "nrg_syev_batched")
+        return info_np
+
+    # ------------------------------------------------------------------ grouped triple products
+    def gemm3_grouped(self, terms, precision=NRG_GEMM_FP64, stream=None):
+        arr = (Gemm3Term * max(len(terms), 1))(*terms)
+        r = lib().nrg_gemm3_grouped(self._h, arr, len(terms), int(precision), _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_gemm3_grouped")
+
+    # ------------------------------------------------------------------ FDM binning
+    def fdm_bin(self, pairs, grid, T, wnf, sign, gt_pos, gt_neg, ls_pos, ls_neg, stream=None):
+        arr = (FdmPair * max(len(pairs), 1))(*pairs)
+        r = lib().nrg_fdm_bin(self._h, arr, len(pairs), ctypes.byref(grid), float(T), float(wnf), int(sign),
+                              ctypes.c_void_p(gt_pos.data_ptr()), ctypes.c_void_p(gt_neg.data_ptr()),
+                              ctypes.c_void_p(ls_pos.data_ptr()), ctypes.c_void_p(ls_neg.data_ptr()),
+                              _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_fdm_bin")
+
+    def kernel_timing(self, enable=True):
+        lib().nrg_kernel_timing(self._h, int(bool(enable)))
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} accumulated since kernel_timing(True)."""
+        buf = ctypes.create_string_buffer(4096)
+        if lib().nrg_kernel_names(self._h, buf, 4096) != 0:
+            return {}
+        out = {}
+        for name in filter(None, buf.value.decode().split(",")):
+            t, c = ctypes.c_double(), ctypes.c_long()
+            lib().nrg_kernel_time(self._h, name.encode(), ctypes.byref(t), ctypes.byref(c))
+            out[name] = (t.value, c.value)
+        return out
+
+    def last_time_ms(self, phase=0):
+        return lib().nrg_last_time_ms(self._h, int(phase))
+
+
+def phase_name(phase):
+    return lib().nrg_phase_name(int(phase)).decode()

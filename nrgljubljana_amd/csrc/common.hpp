@@ -1,0 +1,134 @@
+// common.hpp - shared device helpers and host-side context of the gfx950 NRG core.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+#include "../../include/nrgcore.h"
+
+namespace nrg {
+
+constexpr int WAVE = 64;
+constexpr int NMAX = 4096;   // largest block order accepted by the eigensolver
+
+// ---------------------------------------------------------------- error handling
+void set_error(const std::string &msg);
+#define NRG_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) {                                                                        \
+      ::nrg::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                         \
+      return NRG_E_HIP;                                                                            \
+    }                                                                                              \
+  } while (0)
+
+// ---------------------------------------------------------------- device reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// Sum over a workgroup of NT threads. `red` holds NT/64 entries of LDS. All threads get the sum.
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T v, T *red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; i++) s += red[i];
+  __syncthreads();
+  return s;
+}
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double *red) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int i = 1; i < NT / WAVE; i++) s = fmax(s, red[i]);
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------- host-side device buffer
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);   // grow-only
+  void release();
+  template <typename T> T *as(size_t byte_off = 0) const { return (T *)((char *)p + byte_off); }
+};
+
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+};
+
+constexpr int NPHASE = 8;
+
+// Optional per-launch HIP-event timing of named kernels (off by default).
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Mark { const char *name; size_t b, e; };
+  std::vector<Mark> marks;
+  std::map<std::string, std::pair<double, long>> acc;
+  size_t begin(hipStream_t st);
+  void end(hipStream_t st, size_t b, const char *name);
+  void collect();  // call after the stream has been synchronised
+  void reset();
+  void release();
+};
+
+}  // namespace nrg
+
+// Launch `launch` on `st`, bracketed by timing events when ctx->kt.on.
+#define NRG_TIMED(ctx, st, name, launch)                     \
+  do {                                                        \
+    size_t kb_ = (ctx)->kt.on ? (ctx)->kt.begin(st) : 0;      \
+    launch;                                                   \
+    if ((ctx)->kt.on) (ctx)->kt.end(st, kb_, name);           \
+  } while (0)
+
+struct nrg_ctx {
+  int device = 0;
+  hipEvent_t ev[nrg::NPHASE + 1];
+  float phase_ms[nrg::NPHASE];
+  double total_ms = 0;
+  nrg::DevBuf ws;        // general workspace
+  nrg::DevBuf ws2;       // second workspace (typed arenas of the mixed form)
+  nrg::DevBuf desc;      // descriptor tables
+  nrg::DevBuf flags;     // per-block status words
+  nrg::PinnedBuf hdesc;  // pinned staging of descriptor tables
+  nrg::PinnedBuf hflags;
+  nrg::KTimer kt;
+};
+
+namespace nrg {
+int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
+             double *w, int mode, int *info, hipStream_t st);
+int gemm3_run(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, int precision, hipStream_t st);
+int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf,
+            int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, hipStream_t st);
+// Uploads `bytes` of host data through the pinned staging buffer into ctx->desc at byte offset 0.
+int upload_desc(nrg_ctx *ctx, const void *src, size_t bytes, hipStream_t st);
+}  // namespace nrg

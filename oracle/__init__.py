@@ -1,0 +1,124 @@
+"""CPU oracle of the per-step NRG core -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may import this package,
+and only as the checker. The product library (nrgljubljana_amd) never uses it.
+
+It wraps nrg_oracle.c (a C restatement of the reference's hot path calling the same LAPACK/BLAS
+routines, dsyev and dgemm, that NRG Ljubljana calls) built into liboracle.so by build().
+"""
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(_HERE, "nrg_oracle.c")
+SO = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def blas_path():
+    import scipy
+    libs = glob.glob(os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs",
+                                  "libscipy_openblas*.so"))
+    if not libs:
+        raise RuntimeError("scipy's bundled OpenBLAS was not found")
+    return libs[0]
+
+
+def build(force=False):
+    if os.path.exists(SO) and not force and os.path.getmtime(SO) >= os.path.getmtime(SRC):
+        return SO
+    cmd = ["gcc", "-O2", "-fopenmp", "-shared", "-fPIC", "-std=gnu11", SRC, "-o", SO, "-ldl", "-lm"]
+    subprocess.check_call(cmd)
+    return SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        from nrgljubljana_amd.core import Gemm3Term, FdmPair, BinGrid  # shared struct layouts only
+        L = ctypes.CDLL(SO)
+        L.oracle_init.argtypes = [ctypes.c_char_p]
+        L.oracle_init.restype = ctypes.c_int
+        L.oracle_blas_threads.argtypes = [ctypes.c_int]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_syev_batched.argtypes = [ctypes.c_int, i64p, i64p, dp, i64p, dp, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]
+        L.oracle_syev_batched.restype = ctypes.c_int
+        L.oracle_gemm3.argtypes = [ctypes.POINTER(Gemm3Term), ctypes.c_int]
+        L.oracle_gemm3.restype = ctypes.c_int
+        L.oracle_fdm_bin.argtypes = [ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, dp, dp, dp, dp]
+        L.oracle_fdm_bin.restype = ctypes.c_int
+        L.oracle_jacobi_ld.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.oracle_jacobi_ld.restype = ctypes.c_int
+        r = L.oracle_init(blas_path().encode())
+        if r != 0:
+            raise RuntimeError(f"oracle_init failed ({r})")
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def syev_batched(sizes, off_H, arena, off_w, threads=1, blas_threads=1):
+    """dsyev on every block (c++/diag.hpp:61-82). arena is modified in place (eigenvectors as rows).
+    Returns (w, info)."""
+    L = lib()
+    L.oracle_blas_threads(int(blas_threads))
+    sizes = np.ascontiguousarray(sizes, np.int64)
+    off_H = np.ascontiguousarray(off_H, np.int64)
+    off_w = np.ascontiguousarray(off_w, np.int64)
+    assert arena.dtype == np.float64 and arena.flags.c_contiguous
+    w = np.zeros(int(off_w[-1] + sizes[-1]) if len(sizes) else 0, np.float64)
+    info = (ctypes.c_int * max(len(sizes), 1))()
+    L.oracle_syev_batched(len(sizes), _ip(sizes), _ip(off_H), _dp(arena), _ip(off_w), _dp(w), int(threads), info)
+    return w, np.array(info[:len(sizes)])
+
+
+def syev(h_upper, threads=1):
+    """Single block: returns (w ascending, V with eigenvectors as rows)."""
+    n = h_upper.shape[0]
+    a = np.ascontiguousarray(h_upper, np.float64).ravel().copy()
+    w, info = syev_batched([n], [0], a, [0], threads)
+    if info[0] != 0:
+        raise RuntimeError(f"dsyev INFO={info[0]}")
+    return w, a.reshape(n, n)
+
+
+def jacobi_ld(h_upper):
+    """Brute-force long-double Jacobi: (w ascending, V rows)."""
+    n = h_upper.shape[0]
+    h = np.ascontiguousarray(h_upper, np.float64)
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    lib().oracle_jacobi_ld(n, _dp(h), _dp(w), _dp(V))
+    return w, V
+
+
+def gemm3(terms):
+    """terms: list of Gemm3Term pointing at host numpy buffers."""
+    from nrgljubljana_amd.core import Gemm3Term
+    arr = (Gemm3Term * max(len(terms), 1))(*terms)
+    lib().oracle_gemm3(arr, len(terms))
+
+
+def fdm_bin(pairs, grid, T, wnf, sign, nbins):
+    from nrgljubljana_amd.core import FdmPair
+    out = [np.zeros(nbins) for _ in range(4)]
+    arr = (FdmPair * max(len(pairs), 1))(*pairs)
+    lib().oracle_fdm_bin(arr, len(pairs), ctypes.byref(grid), float(T), float(wnf), int(sign),
+                         _dp(out[0]), _dp(out[1]), _dp(out[2]), _dp(out[3]))
+    return out

@@ -1,0 +1,333 @@
+/*
+ * nrg_oracle.c - CPU oracle for the per-step NRG numerical core.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: the tests, the smoke check
+ * and the CPU-baseline leg of the benchmark load it to produce reference results.
+ * The product library (nrgljubljana_amd/csrc) never links or calls it.
+ *
+ * It restates, in plain C, what NRG Ljubljana does on the hot path, calling the
+ * same third-party LAPACK/BLAS routines the reference calls (dsyev, dgemm). The
+ * reference links whatever LAPACK find_package() finds (CMakeLists.txt:201,
+ * unpinned); here the routines come from the OpenBLAS build bundled with scipy
+ * (scipy_dsyev_, scipy_dgemm_), loaded with dlopen at oracle_init().
+ *
+ * Functions and the reference code they follow:
+ *   oracle_syev_batched   c++/diag.hpp:61-82 (diagonalise_dsyev), c++/core.hpp:90-110
+ *                         (OpenMP schedule(dynamic) loop), c++/subspaces.hpp:114-121
+ *                         (tasks sorted by decreasing size)
+ *   oracle_gemm3          c++/numerics_ublas.hpp:108-140 (product / transform / rotate,
+ *                         same evaluation order as the reference)
+ *   oracle_fdm_bin        c++/algo_FDM.hpp:30-60,84-114 (term loops, gt then ls as in
+ *                         Algo_FDM::calc 136-142), c++/numerics.hpp:283-306 (prod_fit),
+ *                         c++/spectrum.hpp:19-28 (ChainBinning::add),
+ *                         c++/bins.hpp:103-132 (Bins::add, add_std)
+ *   oracle_jacobi_ld      brute-force cyclic Jacobi in long double, used only to pin
+ *                         the LAPACK path on small blocks
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "../include/nrgcore.h"
+
+typedef void (*dsyev_t)(const char *, const char *, const int *, double *, const int *, double *, double *,
+                        const int *, int *, size_t, size_t);
+typedef void (*dgemm_t)(const char *, const char *, const int *, const int *, const int *, const double *,
+                        const double *, const int *, const double *, const int *, const double *, double *,
+                        const int *, size_t, size_t);
+typedef void (*setthreads_t)(int);
+
+static dsyev_t p_dsyev = 0;
+static dgemm_t p_dgemm = 0;
+static setthreads_t p_setthreads = 0;
+
+int oracle_init(const char *blas_path) {
+  if (p_dsyev && p_dgemm) return 0;
+  void *h = dlopen(blas_path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return 1;
+  p_dsyev = (dsyev_t)dlsym(h, "scipy_dsyev_");
+  p_dgemm = (dgemm_t)dlsym(h, "scipy_dgemm_");
+  p_setthreads = (setthreads_t)dlsym(h, "scipy_openblas_set_num_threads");
+  if (!p_dsyev || !p_dgemm) return 2;
+  return 0;
+}
+
+void oracle_blas_threads(int n) {
+  if (p_setthreads) p_setthreads(n);
+}
+
+/* ---------------------------------------------------------------------------
+ * Eigensolver: diagonalise_dsyev (c++/diag.hpp:61-82). The row-major upper
+ * triangle is handed to LAPACK as UPLO='L' column-major (same memory); dsyev
+ * returns eigenvectors as columns of the column-major array, i.e. as ROWS of the
+ * row-major block, which is what copy_vec keeps (c++/diag.hpp:41-48).
+ * ------------------------------------------------------------------------- */
+static int syev_one(int n, double *a, double *w) {
+  const char jobz = 'V', uplo = 'L';
+  int lwork = -1, info = 0;
+  double wq = 0;
+  p_dsyev(&jobz, &uplo, &n, a, &n, w, &wq, &lwork, &info, 1, 1);
+  if (info != 0) return info;
+  lwork = (int)wq;
+  double *work = (double *)malloc(sizeof(double) * (size_t)(lwork > 1 ? lwork : 1));
+  p_dsyev(&jobz, &uplo, &n, a, &n, w, work, &lwork, &info, 1, 1);
+  free(work);
+  return info;
+}
+
+typedef struct { int64_t n; int idx; } task_t;
+static int task_cmp(const void *a, const void *b) {
+  const task_t *x = (const task_t *)a, *y = (const task_t *)b;
+  if (x->n != y->n) return x->n < y->n ? 1 : -1; /* decreasing size, as TaskList */
+  return x->idx - y->idx;
+}
+
+int oracle_syev_batched(int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
+                        double *w, int threads, int *info) {
+  if (!p_dsyev) return -1;
+  task_t *tasks = (task_t *)malloc(sizeof(task_t) * (size_t)(nblk > 0 ? nblk : 1));
+  for (int b = 0; b < nblk; b++) { tasks[b].n = n[b]; tasks[b].idx = b; }
+  qsort(tasks, (size_t)nblk, sizeof(task_t), task_cmp);
+  int nfail = 0;
+  int t;
+#pragma omp parallel for schedule(dynamic) num_threads(threads > 0 ? threads : 1) reduction(+ : nfail)
+  for (t = 0; t < nblk; t++) {
+    const int b = tasks[t].idx;
+    const int r = syev_one((int)n[b], H + off_H[b], w + off_w[b]);
+    if (info) info[b] = r;
+    if (r != 0) nfail++;
+  }
+  free(tasks);
+  return nfail;
+}
+
+/* ---------------------------------------------------------------------------
+ * Row-major dgemm: C[m x n] = alpha * op(A) * op(B) + beta * C, done as the
+ * column-major product C^T = op(B)^T op(A)^T (the reference reaches cblas_dgemm
+ * through Boost bindings on row-major ublas matrices, c++/numerics_ublas.hpp).
+ * ------------------------------------------------------------------------- */
+static void gemm_rm(int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                    const double *B, int64_t ldb, double beta, double *C, int64_t ldc) {
+  if (m == 0 || n == 0) return;
+  const char cta = tb ? 'T' : 'N', ctb = ta ? 'T' : 'N';
+  const int im = (int)n, in = (int)m, ik = (int)k, ilda = (int)ldb, ildb = (int)lda, ildc = (int)ldc;
+  if (k == 0) {
+    for (int64_t i = 0; i < m; i++)
+      for (int64_t j = 0; j < n; j++) C[i * ldc + j] *= beta;
+    return;
+  }
+  p_dgemm(&cta, &ctb, &im, &in, &ik, &alpha, B, &ilda, A, &ildb, &beta, C, &ildc, 1, 1);
+}
+
+/* One term, in the reference's evaluation order:
+ *   rotate    (transA): T = U^T O ; M += f T U            (c++/numerics_ublas.hpp:136-138)
+ *   transform         : T = O B^T ; M += f A T            (c++/numerics_ublas.hpp:124-126)
+ *   product  (O NULL) : M += f A B^T                      (c++/numerics_ublas.hpp:114)   */
+static void gemm3_one(const nrg_gemm3_term *t) {
+  if (t->m == 0 || t->n == 0 || t->k == 0 || t->l == 0) return; /* finite_size() */
+  if (!t->O) {
+    gemm_rm(t->transA, !t->transB, t->m, t->n, t->k, t->f, t->A, t->lda, t->B, t->ldb, 1.0, t->C, t->ldc);
+    return;
+  }
+  if (t->transA) {
+    double *T = (double *)malloc(sizeof(double) * (size_t)(t->m * t->l));
+    gemm_rm(1, 0, t->m, t->l, t->k, 1.0, t->A, t->lda, t->O, t->ldo, 0.0, T, t->l);
+    gemm_rm(0, !t->transB, t->m, t->n, t->l, t->f, T, t->l, t->B, t->ldb, 1.0, t->C, t->ldc);
+    free(T);
+  } else {
+    double *T = (double *)malloc(sizeof(double) * (size_t)(t->k * t->n));
+    gemm_rm(0, !t->transB, t->k, t->n, t->l, 1.0, t->O, t->ldo, t->B, t->ldb, 0.0, T, t->n);
+    gemm_rm(0, 0, t->m, t->n, t->k, t->f, t->A, t->lda, T, t->n, 1.0, t->C, t->ldc);
+    free(T);
+  }
+}
+
+int oracle_gemm3(const nrg_gemm3_term *terms, int nterms) {
+  if (!p_dgemm) return -1;
+  for (int i = 0; i < nterms; i++) gemm3_one(&terms[i]);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * FDM binning.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const nrg_bin_grid *g;
+  double *pos, *neg;
+} chain_t;
+
+/* Bins::add + Bins::add_std (c++/bins.hpp:103-132), zero_epsilon = 1e-14 (bins.hpp:46) */
+static void bins_add(const nrg_bin_grid *g, double *bins, double energy, double weight) {
+  if (fabs(weight) < g->discard_immediately * energy) return;
+  if (energy < 1e-14) { bins[0] += weight; return; }
+  const double x = (log10(energy) - g->log10emin) * g->bins_per_decade;
+  const double ip = floor(x);
+  if (ip < 0) { bins[0] += weight; return; }
+  const size_t index = (size_t)ip;
+  if (index >= (size_t)g->nbins - 1) { bins[g->nbins - 1] += weight; return; }
+  const double rem = x - ip;
+  bins[index] += (1.0 - rem) * weight;
+  bins[index + 1] += rem * weight;
+}
+
+/* ChainBinning::add (c++/spectrum.hpp:19-28) */
+static void chain_add(chain_t *c, double energy, double weight, double factor) {
+  const double w = factor * weight;
+  if (energy >= 0.0) bins_add(c->g, c->pos, energy, w);
+  else bins_add(c->g, c->neg, -energy, w);
+}
+
+/* prod_fit(A, B) (c++/numerics.hpp:283-306) for square A (rho, nr x nr) on the left:
+ * R = rho[:, :rows] * B[:rows, :] with rows = min(nr, dim_j) -> rows x dim_i */
+static double *rho_times_op(const double *rho, int64_t nr, const double *op, int64_t ld, int64_t dimj, int64_t dimi,
+                            int64_t *rows_out) {
+  const int64_t rows = nr < dimj ? nr : dimj;
+  *rows_out = rows;
+  double *R = (double *)calloc((size_t)(rows * dimi) + 1, sizeof(double));
+  if (rows > 0 && dimi > 0) {
+    if (nr <= dimj) /* prod_fit_right: rho * op[0:nr, :] */
+      gemm_rm(0, 0, nr, dimi, nr, 1.0, rho, nr, op, ld, 0.0, R, dimi);
+    else /* prod_fit_left: rho[:, 0:dimj] * op */
+      gemm_rm(0, 0, dimj, dimi, dimj, 1.0, rho, nr, op, ld, 0.0, R, dimi); /* only first dimj rows are used */
+  }
+  return R;
+}
+
+/* op * rho for square rho on the right: R = op[:, :cols] * rho[:cols, :cols] -> dim_j x cols */
+static double *op_times_rho(const double *op, int64_t ld, int64_t dimj, int64_t dimi, const double *rho, int64_t nr,
+                            int64_t *cols_out) {
+  const int64_t cols = nr < dimi ? nr : dimi;
+  *cols_out = cols;
+  double *R = (double *)calloc((size_t)(dimj * cols) + 1, sizeof(double));
+  if (cols > 0 && dimj > 0) gemm_rm(0, 0, dimj, cols, cols, 1.0, op, ld, rho, nr, 0.0, R, cols);
+  return R;
+}
+
+int oracle_fdm_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf, int sign,
+                   double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg) {
+  if (!p_dgemm) return -1;
+  chain_t gt = {grid, gt_pos, gt_neg}, ls = {grid, ls_pos, ls_neg};
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair *q = &pairs[p];
+    const int64_t di = q->dim_i, dj = q->dim_j, Ki = q->K_i, Kj = q->K_j, ld = q->ld_op;
+    const double *op1 = q->op1, *op2 = q->op2;
+#define OP1(j, i) op1[(j) * ld + (i)]
+#define OP2(j, i) op2[(j) * ld + (i)]
+    /* Algo_FDMgt::calc (c++/algo_FDM.hpp:84-114) */
+    {
+      int64_t cols = 0;
+      double *op2_rho = op_times_rho(op2, ld, dj, di, q->rho_i, q->n_rho_i, &cols);
+      for (int64_t i = Ki; i < di; i++)
+        for (int64_t j = Kj; j < dj; j++) {
+          const double Ei = q->E_i[i], Ej = q->E_j[j];
+          chain_add(&gt, Ej - Ei, OP1(j, i) * OP2(j, i) * exp(-Ei / T) * wnf, q->factor);
+        }
+      for (int64_t i = Ki; i < di; i++)
+        for (int64_t j = 0; j < Kj; j++) {
+          const double Ei = q->E_i[i], Ej = q->E_j[j];
+          chain_add(&gt, Ej - Ei, OP1(j, i) * OP2(j, i) * exp(-Ei / T) * wnf, q->factor);
+        }
+      for (int64_t i = 0; i < Ki; i++)
+        for (int64_t j = Kj; j < dj; j++) {
+          const double Ei = q->E_i[i], Ej = q->E_j[j];
+          chain_add(&gt, Ej - Ei, OP1(j, i) * op2_rho[j * cols + i], q->factor);
+        }
+      free(op2_rho);
+    }
+    /* Algo_FDMls::calc (c++/algo_FDM.hpp:30-60) */
+    {
+      int64_t rows = 0;
+      double *rho_op2 = rho_times_op(q->rho_j, q->n_rho_j, op2, ld, dj, di, &rows);
+      for (int64_t i = Ki; i < di; i++)
+        for (int64_t j = Kj; j < dj; j++) {
+          const double Ei = q->E_i[i], Ej = q->E_j[j];
+          chain_add(&ls, Ej - Ei, OP1(j, i) * OP2(j, i) * (-sign) * exp(-Ej / T) * wnf, q->factor);
+        }
+      for (int64_t i = Ki; i < di; i++)
+        for (int64_t j = 0; j < Kj; j++) {
+          const double Ei = q->E_i[i], Ej = q->E_j[j];
+          chain_add(&ls, Ej - Ei, OP1(j, i) * rho_op2[j * di + i] * (-sign), q->factor);
+        }
+      for (int64_t i = 0; i < Ki; i++)
+        for (int64_t j = Kj; j < dj; j++) {
+          const double Ei = q->E_i[i], Ej = q->E_j[j];
+          chain_add(&ls, Ej - Ei, OP1(j, i) * OP2(j, i) * (-sign) * exp(-Ej / T) * wnf, q->factor);
+        }
+      free(rho_op2);
+    }
+#undef OP1
+#undef OP2
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Brute-force cyclic Jacobi in long double (row-major upper triangle in, full
+ * symmetric working copy). Eigenvalues ascending into w, eigenvectors as rows of
+ * V (n x n, row-major). Used only to pin the LAPACK path on small blocks.
+ * ------------------------------------------------------------------------- */
+int oracle_jacobi_ld(int n, const double *Hup, double *w, double *V) {
+  long double *a = (long double *)malloc(sizeof(long double) * (size_t)n * n);
+  long double *v = (long double *)malloc(sizeof(long double) * (size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      a[i * n + j] = (j >= i) ? Hup[i * n + j] : Hup[j * n + i];
+      v[i * n + j] = (i == j) ? 1.0L : 0.0L;
+    }
+  int sweep;
+  for (sweep = 0; sweep < 100; sweep++) {
+    long double off = 0, tot = 0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        tot += a[i * n + j] * a[i * n + j];
+        if (i != j) off += a[i * n + j] * a[i * n + j];
+      }
+    if (off <= tot * 1e-40L || off == 0) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const long double apq = a[p * n + q];
+        if (apq == 0) continue;
+        const long double theta = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+        const long double t = (theta >= 0 ? 1.0L : -1.0L) / (fabsl(theta) + sqrtl(1 + theta * theta));
+        const long double c = 1 / sqrtl(1 + t * t), s = t * c;
+        for (int k = 0; k < n; k++) { /* columns p,q */
+          const long double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) { /* rows p,q */
+          const long double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) { /* V columns */
+          const long double vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = c * vkp - s * vkq;
+          v[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  /* sort ascending: selection sort on the diagonal */
+  int *perm = (int *)malloc(sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  for (int i = 0; i < n; i++) {
+    int best = i;
+    for (int j = i + 1; j < n; j++)
+      if (a[perm[j] * n + perm[j]] < a[perm[best] * n + perm[best]]) best = j;
+    const int tmp = perm[i]; perm[i] = perm[best]; perm[best] = tmp;
+  }
+  for (int i = 0; i < n; i++) {
+    w[i] = (double)a[perm[i] * n + perm[i]];
+    for (int k = 0; k < n; k++) V[i * n + k] = (double)v[k * n + perm[i]];
+  }
+  free(perm);
+  free(a);
+  free(v);
+  return sweep;
+}
