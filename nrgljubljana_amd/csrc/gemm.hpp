@@ -1,0 +1,48 @@
+// gemm.hpp - batched FP64 matrix products on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// One launch evaluates a list of independent products (column-major, BLAS conventions)
+//   C = alpha * (op(A) op(B) + op(A2) op(B2)) + beta * C        (the A2/B2 part only if K2 > 0)
+// tiled in 64x64 blocks of C; work items are (descriptor, tile) pairs.
+#pragma once
+#include "common.hpp"
+
+namespace nrg {
+
+// One K-part of a grouped product: scale * op(A) op(B), op(A) M x K, op(B) K x N.
+struct GemmPart {
+  const double *A, *B;
+  int lda, ldb, tA, tB, K;
+  double scale;
+};
+
+struct GemmDesc {
+  const double *A, *B, *A2, *B2;
+  double *C;
+  int M, N, K, K2;
+  int lda, ldb, lda2, ldb2, ldc;
+  int tA, tB, tA2, tB2;   // 1: use the transpose
+  double alpha, beta;     // beta == 0: C is not read
+  const GemmPart *parts;  // if nparts > 0: C = alpha * sum_p scale_p op(A_p) op(B_p) + beta * C
+  int nparts;
+};
+
+constexpr int GEMM_TILE = 64;
+
+inline int gemm_tiles(const GemmDesc &g) {
+  return ((g.M + GEMM_TILE - 1) / GEMM_TILE) * ((g.N + GEMM_TILE - 1) / GEMM_TILE);
+}
+
+// appends (desc index, tile) items of descriptors [d0, d1) to `items`
+inline void gemm_items(const std::vector<GemmDesc> &d, size_t d0, size_t d1, std::vector<int> &items) {
+  for (size_t i = d0; i < d1; i++) {
+    const int t = gemm_tiles(d[i]);
+    for (int k = 0; k < t; k++) {
+      items.push_back((int)i);
+      items.push_back(k);
+    }
+  }
+}
+
+__global__ void k_gemm_f64(const GemmDesc *descs, const int2 *items);
+
+}  // namespace nrg

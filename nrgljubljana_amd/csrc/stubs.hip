@@ -6,8 +6,8 @@ int syev_mixed_run(nrg_ctx *, int, const int64_t *, const int64_t *, double *, c
   set_error("nrg_syev_batched: mixed form not built yet");
   return NRG_E_ARG;
 }
-int gemm3_run(nrg_ctx *, const nrg_gemm3_term *, int, int, hipStream_t) {
-  set_error("nrg_gemm3_grouped: not built yet");
+int gemm3_ozaki(nrg_ctx *, const nrg_gemm3_term *, int, hipStream_t) {
+  set_error("nrg_gemm3_grouped: split-precision form not built yet");
   return NRG_E_ARG;
 }
 int fdm_run(nrg_ctx *, const nrg_fdm_pair *, int, const nrg_bin_grid *, double, double, int, double *, double *,

@@ -5,14 +5,16 @@
 // All blocks of one NRG step go through the same short sequence of batched launches:
 //
 //   1. k_symm       mirror the stored triangle, giving a full symmetric column-major work matrix
-//   2. k_panel      blocked Householder reduction to tridiagonal form (LAPACK dsytrd/dlatrd
-//                   algorithm, lower variant), one workgroup per block and panel of NB columns;
-//      k_syr2k     trailing rank-2k update A22 -= V W^T + W V^T over 64x64 tiles of every block
+//   2. k_col_prep   blocked Householder reduction to tridiagonal form (LAPACK dsytrd/dlatrd
+//      k_col_symv   algorithm, lower variant): per reduced column, one workgroup per block prepares
+//                   the reflector and many workgroups per block form the symmetric product over
+//                   64x64 tiles of the lower triangle;
+//      k_syr2k      after each panel of NB columns, the trailing rank-2k update A22 -= V W^T + W V^T
 //   3. k_dc_prep    divide and conquer on the tridiagonal matrix (Cuppen's method with the
-//      k_dc_merge   Gu-Eisenstat eigenvector formula, as LAPACK dstedc/dlaed0-3): leaves of order
-//      k_gemm_nn    1-2 at a common tree depth, then one merge launch per depth for all blocks
-//   4. k_bt_tfactor back-transformation V = Q_H Z with compact-WY panels (dlarft + dlarfb)
-//      k_bt_apply
+//      k_dc_*       Gu-Eisenstat eigenvector formula, as LAPACK dstedc/dlaed0-3): leaves of order
+//      k_gemm_f64   1-2 at a common tree depth, then five merge launches per depth for all blocks
+//   4. k_bt_*       back-transformation V = Q_H Z with compact-WY super-panels (dlarft + dlarfb) as
+//      k_gemm_f64   FP64 matrix-core products
 //   5. k_finish     eigenvalues (rescaled) and eigenvectors back into the caller's arenas
 //
 // Storage: a row-major block with its upper triangle filled is, byte for byte, a column-major
@@ -20,6 +22,7 @@
 // A column-major eigenvector matrix with eigenvector i in column i is, byte for byte, the
 // row-major matrix with eigenvector i in row i that copy_vec() builds (c++/diag.hpp:41-48).
 #include "common.hpp"
+#include "gemm.hpp"
 #include <algorithm>
 #include <cstring>
 #include <cmath>
@@ -27,14 +30,11 @@
 namespace nrg {
 
 constexpr int NB = 32;        // panel width of the Householder reduction
-constexpr int NT_PANEL = 512; // threads of the panel kernel
-constexpr int RMAX = (NMAX + NT_PANEL - 1) / NT_PANEL;
-constexpr int NT_DC = 256;    // threads of the merge kernel
+constexpr int NT_PANEL = 512; // threads of the column-preparation kernel
 constexpr int SMALLM = 64;    // merges up to this order do their own block-diagonal product
-constexpr int CW = 32;        // column-strip width of the back-transformation
 
 struct BlkDesc {
-  int64_t n, offA, offZ, offV, offWp, offT, offH, offw;
+  int64_t n, offA, offZ, offV, offWp, offT, offH, offw, offYp;
   int L, P;
 };
 
@@ -45,6 +45,9 @@ struct SyevArgs {
   T *Wp;           // dlatrd W panels, n x NB per block
   T *tau;          // Householder scalars, n per block
   T *Tf;           // compact-WY T factors, P x NB x NB per block
+  T *vb;           // current reflector of every block, n per block
+  T *Yp;           // symmetric-product tile partials, n x ceil(n/64) per block
+  double *Gb, *Tb; // FP64 back-transformation: G_s and T_s of every super-panel
   T *Zt;           // eigenvector matrix used by the back-transformation (== Z0 for FP64)
   double *Z0, *Z1, *S;
   double *d, *e, *lam, *scale;
@@ -52,18 +55,50 @@ struct SyevArgs {
   int *iscr;       // 8 arrays of Ntot ints
   int64_t Ntot;
   int *flags;
+  double *amax;    // per-block max-norm of the input (bit pattern updated with atomicMax)
   double *H, *w;
 };
 
-struct GemmNN {  // C = A*B, column-major, all with leading dimension ld
-  const double *A, *B;
-  double *C;
-  int M, N, K, ld;
-};
 
 // ------------------------------------------------------------------------------------------
 // 1. symmetrisation: lower triangle of the (double) input -> full (T) work matrix
 // ------------------------------------------------------------------------------------------
+// Max-norm of every block's stored triangle (atomicMax on the bit pattern of a non-negative
+// double, whose integer order is the numeric order); non-finite entries give +inf.
+template <typename T>
+__global__ __launch_bounds__(256) void k_absmax(SyevArgs<T> a, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = a.bd[it.x];
+  const int n = (int)bd.n;
+  const int t = it.y;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+  while (ti * (ti + 1) / 2 > t) ti--;
+  const int tj = t - ti * (ti + 1) / 2;
+  const double *Hs = a.H + bd.offH;
+  __shared__ double red[4];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  double mx = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int r = ti * 32 + tx, c = tj * 32 + ty + 8 * q;
+    if (r < n && c < n && r >= c) {
+      const double v = Hs[(size_t)c * n + r];
+      mx = isfinite(v) ? fmax(mx, fabs(v)) : INFINITY;
+    }
+  }
+  mx = block_max<256>(mx, red);
+  if (threadIdx.x == 0 && mx > 0)
+    atomicMax((unsigned long long *)(a.amax + it.x), (unsigned long long)__double_as_longlong(mx));
+}
+
+// Power-of-two factor bringing the block's max-norm into [1, 2): exact, and it keeps squares of
+// the entries away from the subnormal range in the reflector norms (dsyev scales for the same
+// reason when the norm leaves [sqrt(safmin/eps), sqrt(bignum)]).
+__device__ __forceinline__ double block_sigma(double amax) {
+  return (amax > 0 && isfinite(amax)) ? ldexp(1.0, -ilogb(amax)) : 1.0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items) {
   const int2 it = items[blockIdx.x];
@@ -76,6 +111,7 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items) 
   const int tj = t - ti * (ti + 1) / 2;
   const double *Hs = a.H + bd.offH;
   T *Ad = a.A + bd.offA;
+  const double sigma = block_sigma(a.amax[it.x]);
   __shared__ T tile[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
 #pragma unroll
@@ -84,7 +120,7 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items) 
     const int r = ti * 32 + tx, c = tj * 32 + cl;
     T v = 0;
     if (r < n && c < n && r >= c) {
-      v = (T)Hs[(size_t)c * n + r];
+      v = (T)(Hs[(size_t)c * n + r] * sigma);
       Ad[(size_t)c * n + r] = v;
     }
     tile[cl][tx] = v;
@@ -99,19 +135,23 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items) 
 }
 
 // ------------------------------------------------------------------------------------------
-// 2a. panel reduction (dlatrd, lower): one workgroup per block
+// 2a. column preparation (dlatrd, lower): one workgroup per block and reduced column.
+//     Finishes the W column of the previous reflector from the tile partials of k_col_symv,
+//     applies the panel's earlier reflectors to column jl and generates reflector jl.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NT_PANEL) void k_panel(SyevArgs<T> a, const int *blist, int p) {
+__global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int *blist, int p, int jl) {
   const BlkDesc bd = a.bd[blist[blockIdx.x]];
   const int n = (int)bd.n;
   const int i0 = p * NB;
   const int m = n - i0;
-  T *A = a.A + bd.offA + (size_t)i0 * n + i0;
-  T *W = a.Wp + bd.offWp;
+  T *A = a.A + bd.offA + (size_t)i0 * n + i0;   // A(r,c) = A[c*n + r], relative to the panel
+  T *W = a.Wp + bd.offWp;                        // W(r,k) = W[k*n + r]
   double *d = a.d + bd.offV + i0;
   double *e = a.e + bd.offV + i0;
   T *tau = a.tau + bd.offV + i0;
+  T *vb = a.vb + bd.offV + i0;                   // reflector handed to k_col_symv
+  const T *yp = a.Yp + bd.offYp + i0;            // tile partials: yp[slot*n + r]
   const bool final = (m - 1 <= NB);
   const int w = final ? m - 1 : NB;
   const int tid = threadIdx.x;
@@ -122,121 +162,231 @@ __global__ __launch_bounds__(NT_PANEL) void k_panel(SyevArgs<T> a, const int *bl
   __shared__ T vrow[NB], wrow[NB], cw[NB], cv[NB];
   __shared__ T red[NW];
 
-  for (int j = 0; j < w; j++) {
-    if (j > 0) {
-      if (tid < j) {
-        vrow[tid] = A[(size_t)tid * n + j];
-        wrow[tid] = W[(size_t)tid * n + j];
-      }
+  // 1. W(:, jl-1) = tau (A22 v - V (W^T v) - W (V^T v)) - tau/2 (w^T v) v
+  if (jl > 0) {
+    const int jp = jl - 1;
+    const T tv = tau[jp];
+    T *Wj = W + (size_t)jp * n;
+    if (tv != (T)0) {
+      const int t = m - jp - 1, Tt = (t + 63) / 64;
+      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) vs[r] = vb[r];
       __syncthreads();
-      for (int r = j + tid; r < m; r += NT_PANEL) {
-        T acc = 0;
-        for (int k = 0; k < j; k++) acc += A[(size_t)k * n + r] * wrow[k] + W[(size_t)k * n + r] * vrow[k];
-        A[(size_t)j * n + r] -= acc;
+      if (jp > 0) {
+        for (int k = wid; k < jp; k += NW) {
+          T sw = 0, sv = 0, sw2 = 0, sv2 = 0;
+          const T *Wk = W + (size_t)k * n, *Ak = A + (size_t)k * n;
+          int r = jp + 1 + lane;
+          for (; r + WAVE < m; r += 2 * WAVE) {
+            const T w0 = Wk[r], a0 = Ak[r], w1 = Wk[r + WAVE], a1 = Ak[r + WAVE];
+            sw += w0 * vs[r];
+            sv += a0 * vs[r];
+            sw2 += w1 * vs[r + WAVE];
+            sv2 += a1 * vs[r + WAVE];
+          }
+          if (r < m) {
+            sw += Wk[r] * vs[r];
+            sv += Ak[r] * vs[r];
+          }
+          sw += sw2;
+          sv += sv2;
+          sw = wave_sum(sw);
+          sv = wave_sum(sv);
+          if (lane == 0) {
+            cw[k] = sw;
+            cv[k] = sv;
+          }
+        }
+        __syncthreads();
       }
-      __syncthreads();
+      T dp = 0;
+      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) {
+        T y = 0, y2 = 0;
+#pragma unroll 8
+        for (int sl = 0; sl < Tt; sl++) y += yp[(size_t)sl * n + r];
+#pragma unroll 8
+        for (int k = 0; k < jp; k++) {
+          y2 += A[(size_t)k * n + r] * cw[k];
+          y2 += W[(size_t)k * n + r] * cv[k];
+        }
+        y -= y2;
+        y *= tv;
+        Wj[r] = y;
+        dp += y * vs[r];
+      }
+      const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
+      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) Wj[r] += a2 * vs[r];
+    } else {
+      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) Wj[r] = 0;
     }
-    if (j < m - 1) {
-      T *colj = A + (size_t)j * n;
-      const T alpha = colj[j + 1];
-      T part = 0;
-      for (int r = j + 2 + tid; r < m; r += NT_PANEL) part += colj[r] * colj[r];
-      const T xn2 = block_sum<NT_PANEL>(part, red);
-      T beta, tv, scal = 1;
-      if (xn2 == (T)0) {
-        tv = 0;
-        beta = alpha;
-      } else {
-        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-        tv = (beta - alpha) / beta;
-        scal = (T)1 / (alpha - beta);
-      }
-      for (int r = j + 1 + tid; r < m; r += NT_PANEL) {
-        if (r == j + 1) {
-          vs[r] = 1;
-        } else {
-          const T x = colj[r] * scal;
-          colj[r] = x;
-          vs[r] = x;
-        }
-      }
-      __syncthreads();
-      if (tid == 0) {
-        e[j] = (double)beta;
-        tau[j] = tv;
-        colj[j + 1] = 1;
-      }
-      T *Wj = W + (size_t)j * n;
-      if (tv != (T)0) {
-        const int r0 = j + 1 + tid;
-        const int nq = (m - j - 1 + NT_PANEL - 1) / NT_PANEL;
-        T acc[RMAX];
-#pragma unroll
-        for (int q = 0; q < RMAX; q++) acc[q] = 0;
-        // y = A22 * v on the (not yet updated) trailing matrix
-        for (int c = j + 1; c < m; c++) {
-          const T vc = vs[c];
-          const T *col = A + (size_t)c * n;
-#pragma unroll
-          for (int q = 0; q < RMAX; q++) {
-            const int r = r0 + q * NT_PANEL;
-            if (q < nq && r < m) acc[q] += col[r] * vc;
-          }
-        }
-        if (j > 0) {
-          for (int k = wid; k < j; k += NW) {
-            T sw = 0, sv = 0;
-            const T *Wk = W + (size_t)k * n, *Ak = A + (size_t)k * n;
-            for (int r = j + 1 + lane; r < m; r += WAVE) {
-              sw += Wk[r] * vs[r];
-              sv += Ak[r] * vs[r];
-            }
-            sw = wave_sum(sw);
-            sv = wave_sum(sv);
-            if (lane == 0) {
-              cw[k] = sw;
-              cv[k] = sv;
-            }
-          }
-          __syncthreads();
-#pragma unroll
-          for (int q = 0; q < RMAX; q++) {
-            const int r = r0 + q * NT_PANEL;
-            if (q < nq && r < m) {
-              T s = 0;
-              for (int k = 0; k < j; k++) s += A[(size_t)k * n + r] * cw[k] + W[(size_t)k * n + r] * cv[k];
-              acc[q] -= s;
-            }
-          }
-        }
-        T dp = 0;
-#pragma unroll
-        for (int q = 0; q < RMAX; q++) {
-          const int r = r0 + q * NT_PANEL;
-          if (q < nq && r < m) {
-            acc[q] *= tv;
-            dp += acc[q] * vs[r];
-          }
-        }
-        const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
-#pragma unroll
-        for (int q = 0; q < RMAX; q++) {
-          const int r = r0 + q * NT_PANEL;
-          if (q < nq && r < m) Wj[r] = acc[q] + a2 * vs[r];
-        }
-      } else {
-        for (int r = j + 1 + tid; r < m; r += NT_PANEL) Wj[r] = 0;
-      }
-      __syncthreads();
-    }
-    if (tid == 0) d[j] = (double)A[(size_t)j * n + j];
+    __syncthreads();
   }
-  if (final && tid == 0) {
+  if (jl < w) {
+    // 2. apply the panel's earlier reflectors to column jl
+    if (jl > 0) {
+      if (tid < jl) {
+        vrow[tid] = A[(size_t)tid * n + jl];
+        wrow[tid] = W[(size_t)tid * n + jl];
+      }
+      __syncthreads();
+      for (int r = jl + tid; r < m; r += NT_PANEL) {
+        T acc = 0, acc2 = 0;
+#pragma unroll 8
+        for (int k = 0; k < jl; k++) {
+          acc += A[(size_t)k * n + r] * wrow[k];
+          acc2 += W[(size_t)k * n + r] * vrow[k];
+        }
+        A[(size_t)jl * n + r] -= acc + acc2;
+      }
+      __syncthreads();
+    }
+    // 3. reflector annihilating A(jl+2:m, jl) (dlarfg)
+    T *colj = A + (size_t)jl * n;
+    const T alpha = colj[jl + 1];
+    T part = 0;
+    for (int r = jl + 2 + tid; r < m; r += NT_PANEL) part += colj[r] * colj[r];
+    const T xn2 = block_sum<NT_PANEL>(part, red);
+    T beta, tv, scal = 1;
+    if (xn2 == (T)0) {
+      tv = 0;
+      beta = alpha;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tv = (beta - alpha) / beta;
+      scal = (T)1 / (alpha - beta);
+    }
+    for (int r = jl + 1 + tid; r < m; r += NT_PANEL) {
+      if (r == jl + 1) {
+        vb[r] = 1;
+      } else {
+        const T x = colj[r] * scal;
+        colj[r] = x;
+        vb[r] = x;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      e[jl] = (double)beta;
+      tau[jl] = tv;
+      colj[jl + 1] = 1;
+      d[jl] = (double)colj[jl];
+    }
+  } else if (final && tid == 0) {
+    // last diagonal element of the block: apply the panel's reflectors
     T dd = A[(size_t)(m - 1) * n + (m - 1)];
     for (int k = 0; k < w; k++) dd -= (T)2 * A[(size_t)k * n + (m - 1)] * W[(size_t)k * n + (m - 1)];
     A[(size_t)(m - 1) * n + (m - 1)] = dd;
     d[m - 1] = (double)dd;
   }
+}
+
+// Sum over the 64 lanes of x[c(lane)], c(lane) = 8 b5 + 4 b4 + 2 b3 + b2 (bits of the lane id):
+// a butterfly that halves the number of live values per lane at every exchange (17 shuffles).
+template <typename T>
+__device__ __forceinline__ T butterfly16(T (&x)[16]) {
+  const int l = threadIdx.x & (WAVE - 1);
+  {
+    const bool up = l & 32;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const T send = up ? x[j] : x[j + 8];
+      const T keep = up ? x[j + 8] : x[j];
+      x[j] = keep + __shfl_xor(send, 32, WAVE);
+    }
+  }
+  {
+    const bool up = l & 16;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const T send = up ? x[j] : x[j + 4];
+      const T keep = up ? x[j + 4] : x[j];
+      x[j] = keep + __shfl_xor(send, 16, WAVE);
+    }
+  }
+  {
+    const bool up = l & 8;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const T send = up ? x[j] : x[j + 2];
+      const T keep = up ? x[j + 2] : x[j];
+      x[j] = keep + __shfl_xor(send, 8, WAVE);
+    }
+  }
+  {
+    const bool up = l & 4;
+    const T send = up ? x[0] : x[1];
+    const T keep = up ? x[1] : x[0];
+    x[0] = keep + __shfl_xor(send, 4, WAVE);
+  }
+  T s = x[0];
+  s += __shfl_xor(s, 2, WAVE);
+  s += __shfl_xor(s, 1, WAVE);
+  return s;
+}
+
+// ------------------------------------------------------------------------------------------
+// 2b. symmetric matrix-vector product y = A22 v of reduced column jl over 64x64 tiles of the
+//     lower triangle of every block's trailing matrix. Tile (I,J), I >= J, writes A_IJ v_J into
+//     slot J of row segment I and A_IJ^T v_I into slot I of row segment J: every slot of every
+//     segment is written exactly once, so the sum k_col_prep forms is deterministic.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blist, const int *tpre, int nact, int p,
+                                                  int jl) {
+  // block of this workgroup: largest i with tpre[i] <= blockIdx.x
+  int lo = 0, hi = nact - 1;
+  const int g = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tpre[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  const BlkDesc bd = a.bd[blist[lo]];
+  const int tl = g - tpre[lo];
+  int I = (int)((sqrt(8.0 * tl + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= tl) I++;
+  while (I * (I + 1) / 2 > tl) I--;
+  const int J = tl - I * (I + 1) / 2;
+  const int n = (int)bd.n;
+  const int i0 = p * NB;
+  const int m = n - i0;
+  const int r0 = jl + 1;                          // trailing rows/cols [r0, m) of the panel
+  const int t = m - r0;
+  const T *S = a.A + bd.offA + (size_t)(i0 + r0) * n + (i0 + r0);   // S(r,c) = S[c*n + r]
+  const T *v = a.vb + bd.offV + i0 + r0;
+  T *yp = a.Yp + bd.offYp + i0 + r0;             // yp[slot*n + r]
+  __shared__ T vI[64], vJ[64];
+  __shared__ T red[4][64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid < 64) {
+    const int r = I * 64 + tid;
+    vI[tid] = r < t ? v[r] : (T)0;
+  } else if (tid < 128) {
+    const int c = J * 64 + tid - 64;
+    vJ[tid - 64] = c < t ? v[c] : (T)0;
+  }
+  __syncthreads();
+  const int r = I * 64 + lane;
+  const bool rin = r < t;
+  T x[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = J * 64 + wv * 16 + q;
+    x[q] = (rin && c < t) ? S[(size_t)c * n + r] : (T)0;
+  }
+  T yi = 0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) yi += x[q] * vJ[wv * 16 + q];
+  red[wv][lane] = yi;
+  if (I != J) {
+    const T vr = vI[lane];
+#pragma unroll
+    for (int q = 0; q < 16; q++) x[q] *= vr;
+    const T yj = butterfly16(x);
+    const int cl = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+    const int c = J * 64 + wv * 16 + cl;
+    if ((lane & 3) == 0 && c < t) yp[(size_t)I * n + c] = yj;
+  }
+  __syncthreads();
+  if (tid < 64 && rin) yp[(size_t)J * n + r] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -371,30 +521,85 @@ __global__ __launch_bounds__(256) void k_dc_prep(SyevArgs<double> a, const int *
 }
 
 // ------------------------------------------------------------------------------------------
-// 3b. divide and conquer: one merge (secular equation + Loewner eigenvectors), one workgroup
+// 3b. divide and conquer merges. Each merge of two sorted spectra with the rank-one tear
+//     rho z z^T (dlaed1) runs as five launches over all merges of one tree depth:
+//       k_dc_deflate   merge order, deflation (dlaed2), one workgroup per merge
+//       k_dc_roots     secular equation roots (dlaed4), 256 roots per workgroup
+//       k_dc_zhat      Loewner recomputation of z (Gu-Eisenstat, dlaed3)
+//       k_dc_normrank  eigenvector norms and the ascending order of all eigenvalues
+//       k_dc_fill      eigenvector matrix S of the merge (+ undoing the deflation rotations),
+//                      and Qout = diag(Q1, Q2) S for small merges
 // ------------------------------------------------------------------------------------------
+struct MergeState {
+  int k, nrot;
+  double rho;
+};
+enum { DS_DS = 0, DS_ZS, DS_DCAT, DS_ZCAT, DS_DND, DS_ZND, DS_ZHAT, DS_RT, DS_DORG, DS_INVN, DS_RC, DS_RS, DS_COUNT };
+enum { IS_PERM = 0, IS_PINV, IS_CMP, IS_CPOS, IS_COLSRC, IS_RPJ, IS_RNJ, IS_RO, IS_COUNT };
+
+struct NodeGeo {
+  int a0, mid, a1, m1, m2, m;
+};
+__device__ __forceinline__ NodeGeo node_geo(int n, int depth, int q) {
+  NodeGeo g;
+  g.a0 = (int)(((int64_t)q * n) >> depth);
+  g.mid = (int)(((int64_t)(2 * q + 1) * n) >> (depth + 1));
+  g.a1 = (int)(((int64_t)(q + 1) * n) >> depth);
+  g.m1 = g.mid - g.a0;
+  g.m2 = g.a1 - g.mid;
+  g.m = g.m1 + g.m2;
+  return g;
+}
+__device__ __forceinline__ double *dsa(const SyevArgs<double> &a, const BlkDesc &bd, int which, int a0) {
+  return a.dscr + (size_t)which * a.Ntot + bd.offV + a0;
+}
+__device__ __forceinline__ int *isa(const SyevArgs<double> &a, const BlkDesc &bd, int which, int a0) {
+  return a.iscr + (size_t)which * a.Ntot + bd.offV + a0;
+}
+
 // Root j of 1 + rho sum_i z_i^2 / (d_i - lambda) = 0, d ascending, z nonzero, rho > 0.
-// The origin is the nearer pole o; lambda = d_o + tau. Newton iteration on the function with
-// both adjacent poles multiplied out, safeguarded by bisection on the bracket.
+// The origin is the nearer pole o (lambda = d_o + tau); the first iterate solves the model with
+// the two adjacent poles exact and the rest frozen; then Newton on the function with both
+// adjacent poles multiplied out, safeguarded by bisection.
 __device__ void secular_root(int j, int k, const double *d, const double *z, double rho, double zz, int &org,
                              double &tau_out) {
   const double EPS = 2.220446049250313e-16;
   int o, o2;
-  double lo, hi;
+  double lo, hi, t, za, zb = 0.0, d2 = 0.0;
   if (j < k - 1) {
     const double g = d[j + 1] - d[j], mid = 0.5 * g;
-    double f = 1.0;
-    for (int i = 0; i < k; i++) f += rho * z[i] * z[i] / ((d[i] - d[j]) - mid);
+    double R0 = 0.0;
+    for (int i = 0; i < k; i++) {
+      if (i == j || i == j + 1) continue;
+      R0 += rho * z[i] * z[i] / ((d[i] - d[j]) - mid);
+    }
+    const double f = 1.0 + R0 - rho * z[j] * z[j] / mid + rho * z[j + 1] * z[j + 1] / (g - mid);
     if (f >= 0) { o = j; o2 = j + 1; lo = 0; hi = mid; }
     else { o = j + 1; o2 = j; lo = -mid; hi = 0; }
+    d2 = d[o2] - d[o];
+    za = rho * z[o] * z[o];
+    zb = rho * z[o2] * z[o2];
+    t = 0.5 * (lo + hi);
+    const double c = 1.0 + R0, B = c * d2 + za + zb, disc = B * B - 4.0 * c * za * d2;
+    if (disc >= 0) {
+      const double q = 0.5 * (B + copysign(sqrt(disc), B));
+      const double t1 = c != 0 ? q / c : 0.0, t2 = q != 0 ? za * d2 / q : 0.0;
+      if (t1 >= lo && t1 <= hi && t1 != 0.0) t = t1;
+      else if (t2 >= lo && t2 <= hi && t2 != 0.0) t = t2;
+    }
   } else {
     o = k - 1; o2 = -1; lo = 0; hi = rho * zz;
+    za = rho * z[o] * z[o];
+    const double t0 = 0.5 * hi;
+    double R0 = 0.0;
+    for (int i = 0; i < k - 1; i++) R0 += rho * z[i] * z[i] / ((d[i] - d[o]) - t0);
+    t = t0;
+    if (1.0 + R0 > 0) {
+      const double tc = za / (1.0 + R0);
+      if (tc >= lo && tc <= hi && tc != 0.0) t = tc;
+    }
   }
   const double dorg = d[o];
-  const double d2 = o2 >= 0 ? d[o2] - dorg : 0.0;
-  const double za = rho * z[o] * z[o];
-  const double zb = o2 >= 0 ? rho * z[o2] * z[o2] : 0.0;
-  double t = 0.5 * (lo + hi);
   for (int it = 0; it < 100; it++) {
     double R = 0, Rp = 0, Ra = 0;
     for (int i = 0; i < k; i++) {
@@ -446,238 +651,319 @@ __device__ __forceinline__ int upper_bound_d(const double *v, int len, double x)
 }
 __device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v >= hi ? hi - 1 : v); }
 
-__global__ __launch_bounds__(NT_DC) void k_dc_merge(SyevArgs<double> a, const int2 *items, int depth, int mcap) {
+constexpr int NT_DEF = 512;
+
+__global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const int2 *items, int depth,
+                                                       MergeState *mst) {
   extern __shared__ double dyn[];
-  double *dnd = dyn, *znd = dyn + mcap, *rt = dyn + 2 * mcap, *vals = dyn + 3 * mcap;
-  int *ro = (int *)(dyn + 4 * mcap);
-  __shared__ double red[NT_DC / WAVE];
-  __shared__ int sh_k, sh_nrot;
+  __shared__ double red[NT_DEF / WAVE];
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
-  const int n = (int)bd.n, q = it.y, tid = threadIdx.x;
-  const int a0 = (int)(((int64_t)q * n) >> depth);
-  const int mid = (int)(((int64_t)(2 * q + 1) * n) >> (depth + 1));
-  const int a1 = (int)(((int64_t)(q + 1) * n) >> depth);
-  const int m1 = mid - a0, m2 = a1 - mid, m = m1 + m2;
+  const int n = (int)bd.n, tid = threadIdx.x;
+  const NodeGeo g = node_geo(n, depth, it.y);
+  const int m = g.m, m1 = g.m1, m2 = g.m2, a0 = g.a0, mid = g.mid;
+  double *dsl = dyn, *zsl = dyn + m;
   const size_t ld = n;
   const double *Qin = ((depth + 1) & 1 ? a.Z1 : a.Z0) + bd.offZ;
-  double *Qout = (depth & 1 ? a.Z1 : a.Z0) + bd.offZ;
-  double *S = a.S + bd.offZ;
-  double *lam = a.lam + bd.offV;
-  const double *e = a.e + bd.offV;
-  const int64_t N = a.Ntot;
-  double *ds = a.dscr + 0 * N + bd.offV + a0, *zs = a.dscr + 1 * N + bd.offV + a0;
-  double *dcat = a.dscr + 2 * N + bd.offV + a0, *zcat = a.dscr + 3 * N + bd.offV + a0;
-  double *invn = a.dscr + 4 * N + bd.offV + a0, *rc = a.dscr + 5 * N + bd.offV + a0;
-  double *rs = a.dscr + 6 * N + bd.offV + a0;
-  int *perm = a.iscr + 0 * N + bd.offV + a0, *pinv = a.iscr + 1 * N + bd.offV + a0;
-  int *cmp = a.iscr + 2 * N + bd.offV + a0, *cpos = a.iscr + 3 * N + bd.offV + a0;
-  int *colsrc = a.iscr + 4 * N + bd.offV + a0, *rpj = a.iscr + 5 * N + bd.offV + a0;
-  int *rnj = a.iscr + 6 * N + bd.offV + a0;
-
-  const double beta = e[mid - 1];
+  const double *lam = a.lam + bd.offV;
+  const double beta = (a.e + bd.offV)[mid - 1];
   const double sgn = beta >= 0 ? 1.0 : -1.0;
   const double rho = 2.0 * fabs(beta);
   const double r2 = 0.70710678118654752440;
-  // 1. the two sorted spectra and z = (last row of Q1, +-first row of Q2) / sqrt(2)
-  for (int i = tid; i < m; i += NT_DC) {
+  double *dcat = dsa(a, bd, DS_DCAT, a0), *zcat = dsa(a, bd, DS_ZCAT, a0);
+  double *ds = dsa(a, bd, DS_DS, a0);
+  double *dnd = dsa(a, bd, DS_DND, a0), *znd = dsa(a, bd, DS_ZND, a0);
+  double *rc = dsa(a, bd, DS_RC, a0), *rs = dsa(a, bd, DS_RS, a0);
+  int *perm = isa(a, bd, IS_PERM, a0), *pinv = isa(a, bd, IS_PINV, a0);
+  int *cmp = isa(a, bd, IS_CMP, a0), *cpos = isa(a, bd, IS_CPOS, a0);
+  int *rpj = isa(a, bd, IS_RPJ, a0), *rnj = isa(a, bd, IS_RNJ, a0);
+  // the two sorted spectra and z = (last row of Q1, +-first row of Q2) / sqrt(2)
+  for (int i = tid; i < m; i += NT_DEF) {
     dcat[i] = lam[a0 + i];
     zcat[i] = (i < m1 ? Qin[(size_t)(a0 + i) * ld + (mid - 1)] : sgn * Qin[(size_t)(a0 + i) * ld + mid]) * r2;
   }
   __syncthreads();
-  // 2. merge the two sorted lists
   double mx = 0;
-  for (int i = tid; i < m; i += NT_DC) {
+  for (int i = tid; i < m; i += NT_DEF) {
     int pos;
     if (i < m1) pos = i + lower_bound_d(dcat + m1, m2, dcat[i]);
     else pos = (i - m1) + upper_bound_d(dcat, m1, dcat[i]);
     pos = clampi(pos, m);
     perm[pos] = i;
     pinv[i] = pos;
-    ds[pos] = dcat[i];
-    zs[pos] = zcat[i];
+    dsl[pos] = dcat[i];
+    zsl[pos] = zcat[i];
     mx = fmax(mx, fmax(fabs(dcat[i]), fabs(zcat[i])));
   }
-  const double dzmax = block_max<NT_DC>(mx, red);  // includes a barrier
-  // 3. deflation (dlaed2), sequential
+  const double dzmax = block_max<NT_DEF>(mx, red);  // includes a barrier
   if (tid == 0) {
     const double EPS = 2.220446049250313e-16;
     const double tol = 8.0 * EPS * dzmax;
     double zmax = 0;
-    for (int p = 0; p < m; p++) zmax = fmax(zmax, fabs(zs[p]));
+    for (int p = 0; p < m; p++) zmax = fmax(zmax, fabs(zsl[p]));
     int k = 0, nd = 0, nrot = 0;
     if (rho * zmax <= tol) {
       for (int p = 0; p < m; p++) { cmp[p] = -1; cpos[m - 1 - nd] = p; nd++; }
     } else {
       int pj = -1;
       for (int nj = 0; nj < m; nj++) {
-        if (rho * fabs(zs[nj]) <= tol) { cmp[nj] = -1; cpos[m - 1 - nd] = nj; nd++; continue; }
+        const double znj = zsl[nj];
+        if (rho * fabs(znj) <= tol) { cmp[nj] = -1; cpos[m - 1 - nd] = nj; nd++; continue; }
         if (pj < 0) { pj = nj; continue; }
-        double s = zs[pj], c = zs[nj];
-        const double tau = hypot(c, s), t = ds[nj] - ds[pj];
+        double s = zsl[pj], c = znj;
+        const double tau = hypot(c, s), t = dsl[nj] - dsl[pj];
         c /= tau; s = -s / tau;
         if (fabs(t * c * s) <= tol) {
-          zs[nj] = tau; zs[pj] = 0.0;
+          zsl[nj] = tau; zsl[pj] = 0.0;
           rpj[nrot] = pj; rnj[nrot] = nj; rc[nrot] = c; rs[nrot] = s; nrot++;
-          const double t2 = ds[pj] * c * c + ds[nj] * s * s;
-          ds[nj] = ds[pj] * s * s + ds[nj] * c * c;
-          ds[pj] = t2;
+          const double dp = dsl[pj], dn = dsl[nj];
+          dsl[pj] = dp * c * c + dn * s * s;
+          dsl[nj] = dp * s * s + dn * c * c;
           cmp[pj] = -1; cpos[m - 1 - nd] = pj; nd++;
           pj = nj;
         } else {
-          cmp[pj] = k; cpos[k] = pj; dnd[k] = ds[pj]; znd[k] = zs[pj]; k++;
+          cmp[pj] = k; cpos[k] = pj; dnd[k] = dsl[pj]; znd[k] = zsl[pj]; k++;
           pj = nj;
         }
       }
-      if (pj >= 0) { cmp[pj] = k; cpos[k] = pj; dnd[k] = ds[pj]; znd[k] = zs[pj]; k++; }
+      if (pj >= 0) { cmp[pj] = k; cpos[k] = pj; dnd[k] = dsl[pj]; znd[k] = zsl[pj]; k++; }
     }
-    sh_k = k;
-    sh_nrot = nrot;
+    mst[blockIdx.x].k = k;
+    mst[blockIdx.x].nrot = nrot;
+    mst[blockIdx.x].rho = rho;
   }
   __syncthreads();
-  const int k = sh_k, nrot = sh_nrot;
-  // deflated positions were stored from the back: cpos[k..m) holds them in reverse scan order
-  // 4. secular roots
+  for (int p = tid; p < m; p += NT_DEF) ds[p] = dsl[p];
+}
+
+// chunk items: .x = merge index within the depth's merge list, .y = chunk of RCH roots
+constexpr int RCH = 64;
+__global__ __launch_bounds__(RCH) void k_dc_roots(SyevArgs<double> a, const int2 *mitems, const int2 *citems,
+                                                  int depth, const MergeState *mst) {
+  extern __shared__ double dyn[];
+  __shared__ double red[RCH / WAVE];
+  const int2 ci = citems[blockIdx.x];
+  const int2 it = mitems[ci.x];
+  const BlkDesc bd = a.bd[it.x];
+  const NodeGeo g = node_geo((int)bd.n, depth, it.y);
+  const MergeState ms = mst[ci.x];
+  const int k = ms.k;
+  if (ci.y * RCH >= k) return;
+  double *dl = dyn, *zl = dyn + g.m;
+  const double *dnd = dsa(a, bd, DS_DND, g.a0), *znd = dsa(a, bd, DS_ZND, g.a0);
   double zz = 0;
-  for (int i = 0; i < k; i++) zz += znd[i] * znd[i];
-  for (int j = tid; j < k; j += NT_DC) {
+  for (int i = threadIdx.x; i < k; i += RCH) {
+    dl[i] = dnd[i];
+    zl[i] = znd[i];
+    zz += znd[i] * znd[i];
+  }
+  zz = block_sum<RCH>(zz, red);
+  const int j = ci.y * RCH + threadIdx.x;
+  if (j < k) {
     int o; double t;
-    secular_root(j, k, dnd, znd, rho, zz, o, t);
-    ro[j] = o;
-    rt[j] = t;
+    secular_root(j, k, dl, zl, ms.rho, zz, o, t);
+    isa(a, bd, IS_RO, g.a0)[j] = o;
+    dsa(a, bd, DS_RT, g.a0)[j] = t;
+    dsa(a, bd, DS_DORG, g.a0)[j] = dl[o];
+  }
+}
+
+// zhat_i^2 = prod_j (lambda_j - d_i) / prod_{j != i} (d_j - d_i), formed from d_i - lambda_j computed
+// relative to each root's origin
+__global__ __launch_bounds__(256) void k_dc_zhat(SyevArgs<double> a, const int2 *mitems, const int2 *citems, int depth,
+                                                 const MergeState *mst) {
+  extern __shared__ double dyn[];
+  const int2 ci = citems[blockIdx.x];
+  const int2 it = mitems[ci.x];
+  const BlkDesc bd = a.bd[it.x];
+  const NodeGeo g = node_geo((int)bd.n, depth, it.y);
+  const int k = mst[ci.x].k;
+  if (ci.y * 256 >= k) return;
+  double *dl = dyn, *ol = dyn + g.m, *tl = dyn + 2 * g.m;
+  const double *dnd = dsa(a, bd, DS_DND, g.a0), *dorg = dsa(a, bd, DS_DORG, g.a0), *rt = dsa(a, bd, DS_RT, g.a0);
+  for (int i = threadIdx.x; i < k; i += 256) {
+    dl[i] = dnd[i];
+    ol[i] = dorg[i];
+    tl[i] = rt[i];
   }
   __syncthreads();
-  // 5. Loewner: zhat_i^2 = prod_j (lambda_j - d_i) / prod_{j != i} (d_j - d_i)
-  for (int i = tid; i < k; i += NT_DC) {
-    const double di = dnd[i];
-    double W = (di - dnd[ro[i]]) - rt[i];
+  const int i = ci.y * 256 + threadIdx.x;
+  if (i < k) {
+    const double di = dl[i];
+    double W = (di - ol[i]) - tl[i];
     for (int j = 0; j < k; j++) {
       if (j == i) continue;
-      W *= ((di - dnd[ro[j]]) - rt[j]) / (di - dnd[j]);
+      W *= ((di - ol[j]) - tl[j]) / (di - dl[j]);
     }
-    zcat[i] = copysign(sqrt(fabs(W)), znd[i]);
+    dsa(a, bd, DS_ZHAT, g.a0)[i] = copysign(sqrt(fabs(W)), dsa(a, bd, DS_ZND, g.a0)[i]);
+  }
+}
+
+// eigenvector norms of the roots, and ranks of all m eigenvalues of the merge
+__global__ __launch_bounds__(256) void k_dc_normrank(SyevArgs<double> a, const int2 *mitems, const int2 *citems,
+                                                     int depth, const MergeState *mst) {
+  extern __shared__ double dyn[];
+  const int2 ci = citems[blockIdx.x];
+  const int2 it = mitems[ci.x];
+  const BlkDesc bd = a.bd[it.x];
+  const NodeGeo g = node_geo((int)bd.n, depth, it.y);
+  const int m = g.m, k = mst[ci.x].k;
+  if (ci.y * 256 >= m) return;
+  double *dl = dyn, *zl = dyn + m, *vl = dyn + 2 * m;
+  const double *dnd = dsa(a, bd, DS_DND, g.a0), *zhat = dsa(a, bd, DS_ZHAT, g.a0);
+  const double *dorg = dsa(a, bd, DS_DORG, g.a0), *rt = dsa(a, bd, DS_RT, g.a0);
+  const double *ds = dsa(a, bd, DS_DS, g.a0);
+  const int *cpos = isa(a, bd, IS_CPOS, g.a0);
+  for (int i = threadIdx.x; i < m; i += 256) {
+    if (i < k) {
+      dl[i] = dnd[i];
+      zl[i] = zhat[i];
+      vl[i] = dorg[i] + rt[i];
+    } else {
+      vl[i] = ds[clampi(cpos[i], m)];
+    }
   }
   __syncthreads();
-  for (int i = tid; i < k; i += NT_DC) znd[i] = zcat[i];
-  __syncthreads();
-  // 6. norms of the eigenvectors u_ij = zhat_i / (d_i - lambda_j)
-  for (int j = tid; j < k; j += NT_DC) {
-    const double dj = dnd[ro[j]], tj = rt[j];
-    double s = 0;
-    for (int i = 0; i < k; i++) {
-      const double u = znd[i] / ((dnd[i] - dj) - tj);
-      s += u * u;
-    }
-    invn[j] = 1.0 / sqrt(s);
-  }
-  // 7. new eigenvalues and their order
-  for (int u = tid; u < m; u += NT_DC) vals[u] = u < k ? dnd[clampi(ro[u], k)] + rt[u] : ds[clampi(cpos[u], m)];
-  __syncthreads();
-  for (int u = tid; u < m; u += NT_DC) {
-    const double x = vals[u], kx = okey(x);
+  const int u = ci.y * 256 + threadIdx.x;
+  if (u < m) {
+    const double x = vl[u], kx = okey(x);
     int r = 0;
     for (int v = 0; v < m; v++) {
-      const double kv = okey(vals[v]);
+      const double kv = okey(vl[v]);
       r += (kv < kx) || (kv == kx && v < u);
     }
-    colsrc[r] = u;
-    lam[a0 + r] = x;
+    isa(a, bd, IS_COLSRC, g.a0)[r] = u;
+    a.lam[bd.offV + g.a0 + r] = x;
+    if (u < k) {
+      const double dj = dorg[u], tj = rt[u];
+      double s = 0;
+      for (int i = 0; i < k; i++) {
+        const double v = zl[i] / ((dl[i] - dj) - tj);
+        s += v * v;
+      }
+      dsa(a, bd, DS_INVN, g.a0)[u] = 1.0 / sqrt(s);
+    }
+  }
+}
+
+constexpr int FILLC = 32;  // columns of S per fill workgroup
+
+__global__ __launch_bounds__(256) void k_dc_fill(SyevArgs<double> a, const int2 *mitems, const int2 *citems, int depth,
+                                                 const MergeState *mst) {
+  extern __shared__ double dyn[];
+  const int2 ci = citems[blockIdx.x];
+  const int2 it = mitems[ci.x];
+  const BlkDesc bd = a.bd[it.x];
+  const int n = (int)bd.n;
+  const NodeGeo g = node_geo(n, depth, it.y);
+  const int m = g.m, m1 = g.m1, a0 = g.a0;
+  const MergeState ms = mst[ci.x];
+  const int k = ms.k, nrot = ms.nrot;
+  const int c0 = ci.y * FILLC;
+  if (c0 >= m) return;
+  const int nc = min(FILLC, m - c0);
+  const size_t ld = n;
+  double *dl = dyn, *zl = dyn + m;
+  const double *dnd = dsa(a, bd, DS_DND, a0), *zhat = dsa(a, bd, DS_ZHAT, a0);
+  const double *dorg = dsa(a, bd, DS_DORG, a0), *rt = dsa(a, bd, DS_RT, a0), *invn = dsa(a, bd, DS_INVN, a0);
+  const double *rc = dsa(a, bd, DS_RC, a0), *rs = dsa(a, bd, DS_RS, a0);
+  const int *perm = isa(a, bd, IS_PERM, a0), *pinv = isa(a, bd, IS_PINV, a0), *cmp = isa(a, bd, IS_CMP, a0);
+  const int *cpos = isa(a, bd, IS_CPOS, a0), *colsrc = isa(a, bd, IS_COLSRC, a0);
+  const int *rpj = isa(a, bd, IS_RPJ, a0), *rnj = isa(a, bd, IS_RNJ, a0);
+  double *S = a.S + bd.offZ;
+  for (int i = threadIdx.x; i < k; i += 256) {
+    dl[i] = dnd[i];
+    zl[i] = zhat[i];
   }
   __syncthreads();
-  // 8. S in the concatenated row order: column c is the eigenvector of the c-th eigenvalue
-  for (int idx = tid; idx < m * m; idx += NT_DC) {
-    const int c = idx / m, r = idx - c * m;
+  for (int idx = threadIdx.x; idx < nc * m; idx += 256) {
+    const int cc = idx / m, r = idx - cc * m;
+    const int c = c0 + cc;
     const int u = clampi(colsrc[c], m);
     const int p = clampi(pinv[r], m);
     double val;
     if (u < k) {
-      const int ci = cmp[p];
-      val = ci < 0 ? 0.0 : znd[clampi(ci, k)] / ((dnd[clampi(ci, k)] - dnd[clampi(ro[u], k)]) - rt[u]) * invn[u];
+      const int cix = cmp[p];
+      val = cix < 0 ? 0.0 : zl[clampi(cix, k)] / ((dl[clampi(cix, k)] - dorg[u]) - rt[u]) * invn[u];
     } else {
       val = (p == clampi(cpos[u], m)) ? 1.0 : 0.0;
     }
     S[(size_t)(a0 + c) * ld + (a0 + r)] = val;
   }
   __syncthreads();
-  // 9. undo the deflation rotations (last generated first) on the rows of S
-  if (nrot > 0) {
-    for (int c = tid; c < m; c += NT_DC) {
-      double *Sc = S + (size_t)(a0 + c) * ld + a0;
-      for (int t = nrot - 1; t >= 0; t--) {
-        const int r1 = clampi(perm[clampi(rpj[t], m)], m), r2i = clampi(perm[clampi(rnj[t], m)], m);
-        const double cc = rc[t], ss = rs[t];
-        const double x = Sc[r1], y = Sc[r2i];
-        Sc[r1] = cc * x - ss * y;
-        Sc[r2i] = ss * x + cc * y;
-      }
+  // undo the deflation rotations (last generated first) on the rows of this chunk's columns
+  if (nrot > 0 && threadIdx.x < nc) {
+    double *Sc = S + (size_t)(a0 + c0 + threadIdx.x) * ld + a0;
+    for (int t = nrot - 1; t >= 0; t--) {
+      const int r1 = clampi(perm[clampi(rpj[t], m)], m), r2i = clampi(perm[clampi(rnj[t], m)], m);
+      const double cc = rc[t], ss = rs[t];
+      const double x = Sc[r1], y = Sc[r2i];
+      Sc[r1] = cc * x - ss * y;
+      Sc[r2i] = ss * x + cc * y;
     }
-    __syncthreads();
   }
-  // 10. small merges form Qout = diag(Q1, Q2) * S here
+  // small merges: Qout = diag(Q1, Q2) S for this chunk's columns
   if (m <= SMALLM) {
-    for (int idx = tid; idx < m * m; idx += NT_DC) {
-      const int c = idx / m, r = idx - c * m;
-      const double *Sc = S + (size_t)(a0 + c) * ld + a0;
+    __syncthreads();
+    const double *Qin = ((depth + 1) & 1 ? a.Z1 : a.Z0) + bd.offZ;
+    double *Qout = (depth & 1 ? a.Z1 : a.Z0) + bd.offZ;
+    for (int idx = threadIdx.x; idx < nc * m; idx += 256) {
+      const int cc = idx / m, r = idx - cc * m;
+      const double *Sc = S + (size_t)(a0 + c0 + cc) * ld + a0;
       double acc = 0;
       if (r < m1) {
         for (int t = 0; t < m1; t++) acc += Qin[(size_t)(a0 + t) * ld + (a0 + r)] * Sc[t];
       } else {
         for (int t = m1; t < m; t++) acc += Qin[(size_t)(a0 + t) * ld + (a0 + r)] * Sc[t];
       }
-      Qout[(size_t)(a0 + c) * ld + (a0 + r)] = acc;
+      Qout[(size_t)(a0 + c0 + cc) * ld + (a0 + r)] = acc;
     }
   }
 }
 
 // ------------------------------------------------------------------------------------------
-// 3c. C = A * B (column-major FP64), 64x64 tiles, for the large merges
+// 4c. FP64 back-transformation with super-panels of KBT reflectors, on the FP64 matrix cores:
+//     Y_s (explicit, unit lower trapezoidal), G_s = Y_s^T Y_s, T_s by the dlarft recurrence,
+//     Yhat_s = Y_s T_s^T; then for s from last to first: W = Yhat_s^T Z, Z -= Y_s W.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gemm_nn(const GemmNN *probs, const int2 *items) {
+constexpr int KBT = 128;
+
+// Y (column j = reflector j: zeros down to row j, 1 at row j+1, v below), 64 columns per item
+__global__ __launch_bounds__(256) void k_bt_ymat(SyevArgs<double> a, const int2 *items) {
   const int2 it = items[blockIdx.x];
-  const GemmNN P = probs[it.x];
-  const int tm = (P.M + 63) / 64;
-  const int ti = it.y % tm, tj = it.y / tm;
-  __shared__ double As[16][64], Bs[16][65];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  double acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0;
-  for (int k0 = 0; k0 < P.K; k0 += 16) {
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const int idx = tid + 256 * t;
-      const int r = idx & 63, kk = idx >> 6;
-      const int gr = ti * 64 + r, gk = k0 + kk;
-      As[kk][r] = (gr < P.M && gk < P.K) ? P.A[(size_t)gk * P.ld + gr] : 0.0;
-      const int kb = idx & 15, c = idx >> 4;
-      const int gc = tj * 64 + c, gkb = k0 + kb;
-      Bs[kb][c] = (gc < P.N && gkb < P.K) ? P.B[(size_t)gc * P.ld + gkb] : 0.0;
+  const BlkDesc bd = a.bd[it.x];
+  const int n = (int)bd.n;
+  const int j0 = it.y * 64, j1 = min(n - 1, j0 + 64);
+  const double *A = a.A + bd.offA;
+  double *Y = a.S + bd.offZ;
+  for (int j = j0; j < j1; j++)
+    for (int r = threadIdx.x; r < n; r += 256)
+      Y[(size_t)j * n + r] = r == j + 1 ? 1.0 : (r > j + 1 ? A[(size_t)j * n + r] : 0.0);
+}
+
+// T_s (kb x kb, upper triangular, column-major) from G_s and the reflector scalars
+__global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<double> a, const int2 *items, const int64_t *toff) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = a.bd[it.x];
+  const int n = (int)bd.n, sp = it.y;
+  const int kb = min(KBT, (n - 1) - sp * KBT);
+  const size_t off = (size_t)toff[it.x] + (size_t)sp * KBT * KBT;
+  const double *G = a.Gb + off;     // G(c, i) = G[i*kb + c]
+  double *Tg = a.Tb + off;
+  const double *tau = a.tau + bd.offV + sp * KBT;
+  __shared__ double Ts[KBT][KBT + 1];
+  for (int i = 0; i < kb; i++) {
+    const double ti = tau[i];
+    for (int x = threadIdx.x; x < i; x += 256) {
+      double sum = 0;
+      for (int c = x; c < i; c++) sum += Ts[x][c] * G[(size_t)i * kb + c];
+      Ts[x][i] = -ti * sum;
     }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) { av[i] = As[kk][tx + 16 * i]; bv[i] = Bs[kk][ty + 16 * i]; }
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
-    }
+    if (threadIdx.x == 0) Ts[i][i] = ti;
+    for (int x = i + 1 + threadIdx.x; x < kb; x += 256) Ts[x][i] = 0.0;
     __syncthreads();
   }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int gc = tj * 64 + ty + 16 * j;
-    if (gc >= P.N) continue;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int gr = ti * 64 + tx + 16 * i;
-      if (gr < P.M) P.C[(size_t)gc * P.ld + gr] = acc[i][j];
-    }
-  }
+  for (int idx = threadIdx.x; idx < kb * kb; idx += 256) Tg[idx] = Ts[idx % kb][idx / kb];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -725,95 +1011,107 @@ __global__ __launch_bounds__(256) void k_bt_tfactor(SyevArgs<T> a, const int2 *i
 }
 
 // ------------------------------------------------------------------------------------------
-// 4b. V = Q_H Z on a strip of CW columns: Z <- (I - Y T Y^T) Z, last panel first
+// 4b. V = Q_H Z on a strip of bt_cols(n) columns, Z <- (I - Y T Y^T) Z for the panels from last to
+//     first. The strip stays in LDS for the whole sweep; the reflectors stream through 32-row chunks.
 // ------------------------------------------------------------------------------------------
 template <typename T>
+__host__ __device__ constexpr int bt_cols(int n) {
+  return (size_t)n * 16 * sizeof(T) <= 131072 ? 16 : ((size_t)n * 8 * sizeof(T) <= 131072 ? 8 : 4);
+}
+constexpr int BT_LDS = 131072;
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_bt_apply(SyevArgs<T> a, const int2 *items) {
+  __shared__ T Zl[BT_LDS / sizeof(T)];   // strip, column-major with leading dimension n
+  __shared__ T Ys[32][NB + 1];           // [row][reflector]
+  __shared__ T Ts[NB][NB + 1];
+  __shared__ T W1[NB][17], W2[NB][17];
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
   const int n = (int)bd.n;
-  const int c0 = it.y * CW;
-  const int ncol = min(CW, n - c0);
+  const int cw = bt_cols<T>(n);
+  const int c0 = it.y * cw;
+  const int ncol = min(cw, n - c0);
   const T *A = a.A + bd.offA;
   T *Z = a.Zt + bd.offZ;
-  __shared__ T Ys[32][NB + 1];   // [row][reflector]
-  __shared__ T Zs[32][CW + 1];   // [row][column]
-  __shared__ T W1[NB][CW + 1], W2[NB][CW + 1];
-  __shared__ T Ts[NB][NB + 1];
   const int tid = threadIdx.x;
-  const int ta = tid & 31, tc = tid >> 5;  // (reflector or row, column group of 4)
+  for (int idx = tid; idx < n * cw; idx += 256) {
+    const int c = idx / n, r = idx - c * n;
+    Zl[idx] = c < ncol ? Z[(size_t)(c0 + c) * n + r] : (T)0;
+  }
+  const int x = tid & 31, cg = tid >> 5;        // W1 phase: reflector x, columns cg, cg+8
+  const int rr = tid & 31, cq = tid >> 5;       // update phase: row rr of the chunk, columns cq, cq+8
   for (int p = bd.P - 1; p >= 0; p--) {
     const int i0 = p * NB, m = n - i0;
     const int w = (m - 1 <= NB) ? m - 1 : NB;
     const T *Tp = a.Tf + bd.offT + (size_t)p * NB * NB;
+    __syncthreads();
     for (int idx = tid; idx < NB * NB; idx += 256) Ts[idx % NB][idx / NB] = Tp[idx];
-    // W1 = Y^T Z over rows i0+1 .. n-1
-    T acc[4] = {0, 0, 0, 0};
+    T acc0 = 0, acc1 = 0;
     for (int r0 = i0 + 1; r0 < n; r0 += 32) {
       __syncthreads();
       for (int idx = tid; idx < 32 * NB; idx += 256) {
-        const int rr = idx & 31, x = idx >> 5;
-        const int r = r0 + rr;
-        T y = 0;
-        if (r < n && x < w) {
-          const int unit = i0 + x + 1;
-          y = r == unit ? (T)1 : (r > unit ? A[(size_t)(i0 + x) * n + r] : (T)0);
+        const int q = idx & 31, y = idx >> 5;
+        const int r = r0 + q;
+        T v = 0;
+        if (r < n && y < w) {
+          const int unit = i0 + y + 1;
+          v = r == unit ? (T)1 : (r > unit ? A[(size_t)(i0 + y) * n + r] : (T)0);
         }
-        Ys[rr][x] = y;
-      }
-      for (int idx = tid; idx < 32 * CW; idx += 256) {
-        const int rr = idx & 31, c = idx >> 5;
-        const int r = r0 + rr;
-        Zs[rr][c] = (r < n && c < ncol) ? Z[(size_t)(c0 + c) * n + r] : (T)0;
+        Ys[q][y] = v;
       }
       __syncthreads();
-#pragma unroll 8
-      for (int rr = 0; rr < 32; rr++) {
-        const T y = Ys[rr][ta];
-#pragma unroll
-        for (int q = 0; q < 4; q++) acc[q] += y * Zs[rr][tc * 4 + q];
+      const int rmax = min(32, n - r0);
+      const T *z0 = Zl + (size_t)cg * n + r0, *z1 = Zl + (size_t)(cg + 8) * n + r0;
+      if (cg + 8 < cw) {
+        for (int q = 0; q < rmax; q++) {
+          const T yv = Ys[q][x];
+          acc0 += yv * z0[q];
+          acc1 += yv * z1[q];
+        }
+      } else if (cg < cw) {
+        for (int q = 0; q < rmax; q++) acc0 += Ys[q][x] * z0[q];
       }
     }
-#pragma unroll
-    for (int q = 0; q < 4; q++) W1[ta][tc * 4 + q] = acc[q];
+    if (cg < cw) W1[x][cg] = acc0;
+    if (cg + 8 < cw) W1[x][cg + 8] = acc1;
     __syncthreads();
     // W2 = T W1 (T upper triangular)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int c = tc * 4 + q;
-      T s = 0;
-      for (int y = ta; y < NB; y++) s += Ts[ta][y] * W1[y][c];
-      W2[ta][c] = s;
+    for (int idx = tid; idx < NB * cw; idx += 256) {
+      const int y = idx % NB, c = idx / NB;
+      T sum = 0;
+      for (int q = y; q < NB; q++) sum += Ts[y][q] * W1[q][c];
+      W2[y][c] = sum;
     }
     // Z -= Y W2
     for (int r0 = i0 + 1; r0 < n; r0 += 32) {
       __syncthreads();
       for (int idx = tid; idx < 32 * NB; idx += 256) {
-        const int rr = idx & 31, x = idx >> 5;
-        const int r = r0 + rr;
-        T y = 0;
-        if (r < n && x < w) {
-          const int unit = i0 + x + 1;
-          y = r == unit ? (T)1 : (r > unit ? A[(size_t)(i0 + x) * n + r] : (T)0);
+        const int q = idx & 31, y = idx >> 5;
+        const int r = r0 + q;
+        T v = 0;
+        if (r < n && y < w) {
+          const int unit = i0 + y + 1;
+          v = r == unit ? (T)1 : (r > unit ? A[(size_t)(i0 + y) * n + r] : (T)0);
         }
-        Ys[rr][x] = y;
+        Ys[q][y] = v;
       }
       __syncthreads();
-      const int r = r0 + ta;
+      const int r = r0 + rr;
       if (r < n) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int c = tc * 4 + q;
-          if (c < ncol) {
-            T s = 0;
+        for (int c = cq; c < cw; c += 8) {
+          T sum = 0;
 #pragma unroll 8
-            for (int x = 0; x < NB; x++) s += Ys[ta][x] * W2[x][c];
-            Z[(size_t)(c0 + c) * n + r] -= s;
-          }
+          for (int y = 0; y < NB; y++) sum += Ys[rr][y] * W2[y][c];
+          Zl[(size_t)c * n + r] -= sum;
         }
       }
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int idx = tid; idx < n * ncol; idx += 256) {
+    const int c = idx / n, r = idx - c * n;
+    Z[(size_t)(c0 + c) * n + r] = Zl[idx];
   }
 }
 
@@ -825,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_finish(SyevArgs<T> a, const int *blist)
   const int b = blist[blockIdx.x];
   const BlkDesc bd = a.bd[b];
   const int n = (int)bd.n;
-  const double s = a.scale[b];
+  const double s = a.scale[b] / block_sigma(a.amax[b]);
   const double *lam = a.lam + bd.offV;
   double *w = a.w + bd.offw;
   int bad = 0;
@@ -834,11 +1132,21 @@ __global__ __launch_bounds__(256) void k_finish(SyevArgs<T> a, const int *blist)
     w[i] = v;
     bad |= !isfinite(v);
   }
+  if (bad) a.flags[b] = 2;
+}
+
+constexpr int COPY_CHUNK = 16384;
+
+// eigenvectors into the caller's arena: item (block, chunk of COPY_CHUNK elements)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_out(SyevArgs<T> a, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = a.bd[it.x];
+  const size_t nn = (size_t)bd.n * bd.n;
+  const size_t beg = (size_t)it.y * COPY_CHUNK, end = min(nn, beg + COPY_CHUNK);
   const T *Z = a.Zt + bd.offZ;
   double *H = a.H + bd.offH;
-  const size_t nn = (size_t)n * n;
-  for (size_t i = threadIdx.x; i < nn; i += 256) H[i] = (double)Z[i];
-  if (bad) a.flags[b] = 2;
+  for (size_t i = beg + threadIdx.x; i < end; i += 256) H[i] = (double)Z[i];
 }
 
 template <typename T>
@@ -886,12 +1194,26 @@ struct SyevPlan {
   // work lists, all packed into one int array; offsets in ints
   std::vector<int> ibuf;
   size_t symm_off = 0, symm_cnt = 0;
-  std::vector<size_t> pan_off, pan_cnt, upd_off, upd_cnt;
+  struct ColLaunch { int p, jl; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };
+  std::vector<ColLaunch> cols;
+  std::vector<size_t> upd_off, upd_cnt;
+  int64_t sumYp = 0;
   size_t all_off = 0;  // list 0..nblk-1
-  std::vector<size_t> mrg_off, mrg_cnt, gemm_off, gemm_cnt, prob_first, prob_cnt;
-  size_t tf_off = 0, tf_cnt = 0, bt_off = 0, bt_cnt = 0;
+  std::vector<size_t> mrg_off, mrg_cnt, gemm_off, gemm_cnt;
+  std::vector<size_t> ch_off, ch_cnt, fc_off, fc_cnt;  // 256-wide chunks and FILLC-wide column chunks
+  std::vector<size_t> rc_off, rc_cnt;                  // RCH-wide root chunks
+  size_t mrg_max = 0;
+  size_t tf_off = 0, tf_cnt = 0, bt_off = 0, bt_cnt = 0, cp_off = 0, cp_cnt = 0;
   std::vector<int> mcap;
-  std::vector<GemmNN> probs;  // filled once the arena pointers are known
+  std::vector<GemmDesc> gd;   // products, filled once the arena pointers are known
+  // FP64 back-transformation
+  std::vector<int64_t> toff;  // per block: offset of its super-panels in the G/T arenas
+  int64_t sumTb = 0, sumW2 = 0;
+  int Smax = 0;
+  size_t ym_off = 0, ym_cnt = 0, tb_off = 0, tb_cnt = 0;
+  size_t g_items = 0, g_cnt = 0, yh_items = 0, yh_cnt = 0;
+  std::vector<size_t> w_items, w_cnt, u_items, u_cnt;
+  std::vector<size_t> s2_items, s2_cnt;   // FP64 trailing rank-2k updates per panel
 };
 
 static int panels_of(int64_t n) {
@@ -902,7 +1224,7 @@ static int panels_of(int64_t n) {
 }
 
 static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *off_H, const int64_t *off_w,
-                       bool fp64_inplace) {
+                       bool fp64_inplace, int elem_bytes) {
   pl.bd.resize(nblk);
   int64_t zo = 0, vo = 0, wo = 0, to = 0, ao = 0;
   for (int b = 0; b < nblk; b++) {
@@ -914,6 +1236,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     d.offWp = wo; wo += n[b] * NB;
     d.P = panels_of(n[b]);
     d.offT = to; to += (int64_t)d.P * NB * NB;
+    d.offYp = pl.sumYp; pl.sumYp += n[b] * ((n[b] + 63) / 64);
     d.offH = off_H[b];
     d.offw = off_w[b];
     d.L = 31 - __builtin_clz((unsigned)n[b]);
@@ -938,13 +1261,37 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     for (int64_t q = 0; q < t * (t + 1) / 2; q++) push2(b, (int)q);
   }
   pl.symm_cnt = (I.size() - pl.symm_off) / 2;
-  // panels and updates
-  pl.pan_off.assign(pl.Pmax, 0); pl.pan_cnt.assign(pl.Pmax, 0);
+  // reduced columns (two launches each) and trailing updates
+  pl.cols.clear();
   pl.upd_off.assign(pl.Pmax, 0); pl.upd_cnt.assign(pl.Pmax, 0);
   for (int p = 0; p < pl.Pmax; p++) {
-    pl.pan_off[p] = I.size();
-    for (int b : order) if (pl.bd[b].P > p) I.push_back(b);
-    pl.pan_cnt[p] = I.size() - pl.pan_off[p];
+    auto width = [&](int b) {
+      const int64_t m = n[b] - (int64_t)NB * p;
+      return (int)(m - 1 <= NB ? m - 1 : NB);
+    };
+    int wmax = 0;
+    for (int b : order) if (pl.bd[b].P > p) wmax = std::max(wmax, width(b));
+    for (int jl = 0; jl <= wmax; jl++) {
+      SyevPlan::ColLaunch c;
+      c.p = p; c.jl = jl;
+      c.prep_off = I.size();
+      for (int b : order) if (pl.bd[b].P > p && jl <= width(b)) I.push_back(b);
+      c.prep_cnt = I.size() - c.prep_off;
+      c.symv_off = I.size();
+      for (int b : order) if (pl.bd[b].P > p && jl < width(b)) I.push_back(b);
+      c.nact = I.size() - c.symv_off;
+      c.tpre_off = I.size();
+      int64_t tiles = 0;
+      for (size_t q = 0; q < c.nact; q++) {
+        const int b = I[c.symv_off + q];
+        const int64_t t = n[b] - (int64_t)NB * p - jl - 1;
+        const int64_t T = (t + 63) / 64;
+        I.push_back((int)tiles);
+        tiles += T * (T + 1) / 2;
+      }
+      c.tiles = (size_t)tiles;
+      pl.cols.push_back(c);
+    }
     pl.upd_off[p] = I.size();
     for (int b : order)
       if (pl.bd[b].P > p + 1) {
@@ -957,7 +1304,6 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   // divide and conquer merges, deepest first
   pl.mrg_off.assign(pl.Lmax, 0); pl.mrg_cnt.assign(pl.Lmax, 0);
   pl.gemm_off.assign(pl.Lmax, 0); pl.gemm_cnt.assign(pl.Lmax, 0);
-  pl.prob_first.assign(pl.Lmax, 0); pl.prob_cnt.assign(pl.Lmax, 0);
   pl.mcap.assign(pl.Lmax, 0);
   for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
     pl.mrg_off[depth] = I.size();
@@ -971,6 +1317,29 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
         }
     pl.mrg_cnt[depth] = (I.size() - pl.mrg_off[depth]) / 2;
     pl.mcap[depth] = mc;
+    pl.mrg_max = std::max(pl.mrg_max, pl.mrg_cnt[depth]);
+  }
+  pl.ch_off.assign(pl.Lmax, 0); pl.ch_cnt.assign(pl.Lmax, 0);
+  pl.rc_off.assign(pl.Lmax, 0); pl.rc_cnt.assign(pl.Lmax, 0);
+  pl.fc_off.assign(pl.Lmax, 0); pl.fc_cnt.assign(pl.Lmax, 0);
+  for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
+    std::vector<int> sizes_here;
+    for (size_t mi = 0; mi < pl.mrg_cnt[depth]; mi++) {
+      const int b = I[pl.mrg_off[depth] + 2 * mi], q = I[pl.mrg_off[depth] + 2 * mi + 1];
+      sizes_here.push_back((int)((((int64_t)(q + 1) * n[b]) >> depth) - (((int64_t)q * n[b]) >> depth)));
+    }
+    pl.ch_off[depth] = I.size();
+    for (size_t mi = 0; mi < sizes_here.size(); mi++)
+      for (int c = 0; c < (sizes_here[mi] + 255) / 256; c++) push2((int)mi, c);
+    pl.ch_cnt[depth] = (I.size() - pl.ch_off[depth]) / 2;
+    pl.rc_off[depth] = I.size();
+    for (size_t mi = 0; mi < sizes_here.size(); mi++)
+      for (int c = 0; c < (sizes_here[mi] + RCH - 1) / RCH; c++) push2((int)mi, c);
+    pl.rc_cnt[depth] = (I.size() - pl.rc_off[depth]) / 2;
+    pl.fc_off[depth] = I.size();
+    for (size_t mi = 0; mi < sizes_here.size(); mi++)
+      for (int c = 0; c < (sizes_here[mi] + FILLC - 1) / FILLC; c++) push2((int)mi, c);
+    pl.fc_cnt[depth] = (I.size() - pl.fc_off[depth]) / 2;
   }
   // compact-WY factors and back-transformation strips
   pl.tf_off = I.size();
@@ -980,17 +1349,66 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   pl.tf_cnt = (I.size() - pl.tf_off) / 2;
   pl.bt_off = I.size();
   for (int b : order)
-    if (n[b] >= 3)
-      for (int s = 0; s < (n[b] + CW - 1) / CW; s++) push2(b, s);
+    if (n[b] >= 3) {
+      const int cw = elem_bytes == 4 ? bt_cols<float>((int)n[b]) : bt_cols<double>((int)n[b]);
+      for (int s = 0; s < (n[b] + cw - 1) / cw; s++) push2(b, s);
+    }
   pl.bt_cnt = (I.size() - pl.bt_off) / 2;
+  // FP64 back-transformation: super-panel offsets, Y columns and T factors
+  pl.toff.assign(nblk, 0);
+  pl.sumTb = 0; pl.sumW2 = 0; pl.Smax = 0;
+  for (int b = 0; b < nblk; b++) {
+    const int sb = n[b] >= 3 ? (int)((n[b] - 1 + KBT - 1) / KBT) : 0;
+    pl.toff[b] = pl.sumTb;
+    pl.sumTb += (int64_t)sb * KBT * KBT;
+    pl.sumW2 += (int64_t)KBT * n[b];
+    pl.Smax = std::max(pl.Smax, sb);
+  }
+  pl.ym_off = I.size();
+  for (int b : order)
+    if (n[b] >= 3)
+      for (int c = 0; c < (int)((n[b] - 1 + 63) / 64); c++) push2(b, c);
+  pl.ym_cnt = (I.size() - pl.ym_off) / 2;
+  pl.tb_off = I.size();
+  for (int b : order)
+    if (n[b] >= 3)
+      for (int sp = 0; sp < (int)((n[b] - 1 + KBT - 1) / KBT); sp++) push2(b, sp);
+  pl.tb_cnt = (I.size() - pl.tb_off) / 2;
+  pl.cp_off = I.size();
+  for (int b : order)
+    for (int64_t c = 0; c < (n[b] * n[b] + COPY_CHUNK - 1) / COPY_CHUNK; c++) push2(b, (int)c);
+  pl.cp_cnt = (I.size() - pl.cp_off) / 2;
 }
 
-// GEMM problems of the large merges at each depth (pointers into the arenas)
-static void build_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S) {
-  pl.probs.clear();
+// Products (pointers into the arenas): the large D&C merges at each depth and the FP64
+// back-transformation. Appends the work items to pl.ibuf.
+static void build_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S, double *Gb, double *Tb, double *W2,
+                        double *Abase, double *Wpbase) {
+  pl.gd.clear();
+  std::vector<int> &I = pl.ibuf;
+  // trailing updates A22 -= V W^T + W V^T after every panel that is not a block's last
+  pl.s2_items.assign(pl.Pmax, 0); pl.s2_cnt.assign(pl.Pmax, 0);
+  for (int p = 0; p < pl.Pmax; p++) {
+    const size_t d0 = pl.gd.size();
+    for (const BlkDesc &d : pl.bd) {
+      if (d.P <= p + 1) continue;
+      const int n = (int)d.n, i0 = p * NB, mt = n - i0 - NB;
+      GemmDesc g{};
+      double *Ab = Abase + d.offA;
+      const double *V = Ab + (size_t)i0 * n + (i0 + NB), *Wt = Wpbase + d.offWp + NB;
+      g.A = V; g.lda = n; g.tA = 0; g.B = Wt; g.ldb = n; g.tB = 1; g.K = NB;
+      g.A2 = Wt; g.lda2 = n; g.tA2 = 0; g.B2 = V; g.ldb2 = n; g.tB2 = 1; g.K2 = NB;
+      g.C = Ab + (size_t)(i0 + NB) * n + (i0 + NB); g.ldc = n;
+      g.M = mt; g.N = mt; g.alpha = -1.0; g.beta = 1.0;
+      pl.gd.push_back(g);
+    }
+    pl.s2_items[p] = I.size();
+    gemm_items(pl.gd, d0, pl.gd.size(), I);
+    pl.s2_cnt[p] = (I.size() - pl.s2_items[p]) / 2;
+  }
   for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
-    pl.prob_first[depth] = pl.probs.size();
-    const int *it = pl.ibuf.data() + pl.mrg_off[depth];
+    const size_t d0 = pl.gd.size();
+    const int *it = I.data() + pl.mrg_off[depth];
     for (size_t i = 0; i < pl.mrg_cnt[depth]; i++) {
       const int b = it[2 * i], q = it[2 * i + 1];
       const BlkDesc &d = pl.bd[b];
@@ -1006,31 +1424,98 @@ static void build_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S) {
       for (int half = 0; half < 2; half++) {
         const int ro = half ? mid - a0 : 0;
         const int mh = half ? a1 - mid : mid - a0;
-        GemmNN g;
-        g.A = Qin + (size_t)(a0 + ro) * nb + (a0 + ro);
-        g.B = Sb + (size_t)a0 * nb + (a0 + ro);
-        g.C = Qout + (size_t)a0 * nb + (a0 + ro);
-        g.M = mh; g.N = m; g.K = mh; g.ld = nb;
-        pl.probs.push_back(g);
+        GemmDesc g{};
+        g.A = Qin + (size_t)(a0 + ro) * nb + (a0 + ro); g.lda = nb; g.tA = 0;
+        g.B = Sb + (size_t)a0 * nb + (a0 + ro); g.ldb = nb; g.tB = 0;
+        g.C = Qout + (size_t)a0 * nb + (a0 + ro); g.ldc = nb;
+        g.M = mh; g.N = m; g.K = mh; g.K2 = 0; g.alpha = 1.0; g.beta = 0.0;
+        pl.gd.push_back(g);
       }
     }
-    pl.prob_cnt[depth] = pl.probs.size() - pl.prob_first[depth];
+    pl.gemm_off[depth] = I.size();
+    gemm_items(pl.gd, d0, pl.gd.size(), I);
+    pl.gemm_cnt[depth] = (I.size() - pl.gemm_off[depth]) / 2;
   }
-  // tile items for every depth appended to ibuf
-  for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
-    pl.gemm_off[depth] = pl.ibuf.size();
-    for (size_t pi = pl.prob_first[depth]; pi < pl.prob_first[depth] + pl.prob_cnt[depth]; pi++) {
-      const GemmNN &g = pl.probs[pi];
-      const int tiles = ((g.M + 63) / 64) * ((g.N + 63) / 64);
-      for (int t = 0; t < tiles; t++) { pl.ibuf.push_back((int)pi); pl.ibuf.push_back(t); }
+  // back-transformation products
+  const int nblk = (int)pl.bd.size();
+  auto nsp = [&](int b) { const int n = (int)pl.bd[b].n; return n >= 3 ? (n - 1 + KBT - 1) / KBT : 0; };
+  size_t d0 = pl.gd.size();
+  for (int b = 0; b < nblk; b++) {
+    const BlkDesc &d = pl.bd[b];
+    const int n = (int)d.n;
+    for (int sp = 0; sp < nsp(b); sp++) {
+      const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
+      GemmDesc g{};
+      g.A = S + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
+      g.B = g.A; g.ldb = n; g.tB = 0;
+      g.C = Gb + pl.toff[b] + (size_t)sp * KBT * KBT; g.ldc = kb;
+      g.M = kb; g.N = kb; g.K = n - r0; g.alpha = 1.0; g.beta = 0.0;
+      pl.gd.push_back(g);
     }
-    pl.gemm_cnt[depth] = (pl.ibuf.size() - pl.gemm_off[depth]) / 2;
+  }
+  pl.g_items = I.size();
+  gemm_items(pl.gd, d0, pl.gd.size(), I);
+  pl.g_cnt = (I.size() - pl.g_items) / 2;
+  d0 = pl.gd.size();
+  for (int b = 0; b < nblk; b++) {
+    const BlkDesc &d = pl.bd[b];
+    const int n = (int)d.n;
+    for (int sp = 0; sp < nsp(b); sp++) {
+      const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
+      GemmDesc g{};
+      g.A = S + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
+      g.B = Tb + pl.toff[b] + (size_t)sp * KBT * KBT; g.ldb = kb; g.tB = 1;
+      g.C = Z1 + d.offZ + (size_t)sp * KBT * n + r0; g.ldc = n;
+      g.M = n - r0; g.N = kb; g.K = kb; g.alpha = 1.0; g.beta = 0.0;
+      pl.gd.push_back(g);
+    }
+  }
+  pl.yh_items = I.size();
+  gemm_items(pl.gd, d0, pl.gd.size(), I);
+  pl.yh_cnt = (I.size() - pl.yh_items) / 2;
+  pl.w_items.assign(pl.Smax, 0); pl.w_cnt.assign(pl.Smax, 0);
+  pl.u_items.assign(pl.Smax, 0); pl.u_cnt.assign(pl.Smax, 0);
+  for (int step = 0; step < pl.Smax; step++) {
+    for (int phase = 0; phase < 2; phase++) {
+      d0 = pl.gd.size();
+      int64_t w2o = 0;
+      for (int b = 0; b < nblk; b++) {
+        const BlkDesc &d = pl.bd[b];
+        const int n = (int)d.n;
+        const int S_b = nsp(b);
+        double *W2b = W2 + w2o;
+        w2o += (int64_t)KBT * n;
+        if (S_b <= step) continue;
+        const int sp = S_b - 1 - step;
+        const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
+        GemmDesc g{};
+        if (phase == 0) {   // W = Yhat_s^T Z(r0:, :)
+          g.A = Z1 + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
+          g.B = Z0 + d.offZ + r0; g.ldb = n; g.tB = 0;
+          g.C = W2b; g.ldc = kb;
+          g.M = kb; g.N = n; g.K = n - r0; g.alpha = 1.0; g.beta = 0.0;
+        } else {            // Z(r0:, :) -= Y_s W
+          g.A = S + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
+          g.B = W2b; g.ldb = kb; g.tB = 0;
+          g.C = Z0 + d.offZ + r0; g.ldc = n;
+          g.M = n - r0; g.N = n; g.K = kb; g.alpha = -1.0; g.beta = 1.0;
+        }
+        pl.gd.push_back(g);
+      }
+      size_t &io = phase == 0 ? pl.w_items[step] : pl.u_items[step];
+      size_t &ic = phase == 0 ? pl.w_cnt[step] : pl.u_cnt[step];
+      io = I.size();
+      gemm_items(pl.gd, d0, pl.gd.size(), I);
+      ic = (I.size() - io) / 2;
+    }
   }
 }
 
 struct SyevLaunch {
   const int *dI;      // device copy of ibuf
-  const GemmNN *dP;   // device copy of probs
+  const GemmDesc *dG; // device copy of the product descriptors
+  const int64_t *toff; // device copy of the per-block super-panel offsets
+  MergeState *mst;    // per-merge state of the current depth
 };
 
 // Launch sequence of the reduction + D&C + back-transformation for a typed work matrix.
@@ -1041,12 +1526,26 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   auto grid = [](size_t c) { return dim3((unsigned)c); };
   // 1. symmetrisation
   if (pl.symm_cnt)
+    NRG_TIMED(ctx, st, "k_absmax", hipLaunchKernelGGL(k_absmax<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off)));
+  if (pl.symm_cnt)
     NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off)));
   // 2. reduction
+  size_t ci = 0;
   for (int p = 0; p < pl.Pmax; p++) {
-    if (pl.pan_cnt[p])
-      NRG_TIMED(ctx, st, "k_panel", hipLaunchKernelGGL(k_panel<T>, grid(pl.pan_cnt[p]), dim3(NT_PANEL), 0, st, aT, I + pl.pan_off[p], p));
-    if (pl.upd_cnt[p])
+    for (; ci < pl.cols.size() && pl.cols[ci].p == p; ci++) {
+      const SyevPlan::ColLaunch &c = pl.cols[ci];
+      if (c.prep_cnt)
+        NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL(k_col_prep<T>, grid(c.prep_cnt), dim3(NT_PANEL), 0, st, aT,
+                                                            I + c.prep_off, p, c.jl));
+      if (c.tiles)
+        NRG_TIMED(ctx, st, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, st, aT,
+                                                            I + c.symv_off, I + c.tpre_off, (int)c.nact, p, c.jl));
+    }
+    if constexpr (sizeof(T) == 8) {
+      if (pl.s2_cnt[p])
+        NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.s2_cnt[p]), dim3(256), 0, st, L.dG,
+                                                            (const int2 *)(I + pl.s2_items[p])));
+    } else if (pl.upd_cnt[p])
       NRG_TIMED(ctx, st, "k_syr2k", hipLaunchKernelGGL(k_syr2k<T>, grid(pl.upd_cnt[p]), dim3(256), 0, st, aT, (const int2 *)(I + pl.upd_off[p]), p));
   }
   NRG_HIP(hipGetLastError());
@@ -1055,18 +1554,33 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   {
     int mc = 0;
     for (int depth = 0; depth < pl.Lmax; depth++) mc = std::max(mc, pl.mcap[depth]);
-    const size_t lds = (size_t)mc * (4 * sizeof(double) + sizeof(int)) + 16;
-    (void)hipFuncSetAttribute((const void *)k_dc_merge, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int lds3 = (int)(3 * sizeof(double) * (size_t)mc + 16), lds2 = (int)(2 * sizeof(double) * (size_t)mc + 16);
+    (void)hipFuncSetAttribute((const void *)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
+    (void)hipFuncSetAttribute((const void *)k_dc_roots, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
+    (void)hipFuncSetAttribute((const void *)k_dc_zhat, hipFuncAttributeMaxDynamicSharedMemorySize, lds3);
+    (void)hipFuncSetAttribute((const void *)k_dc_normrank, hipFuncAttributeMaxDynamicSharedMemorySize, lds3);
+    (void)hipFuncSetAttribute((const void *)k_dc_fill, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
   }
   NRG_TIMED(ctx, st, "k_dc_prep", hipLaunchKernelGGL(k_dc_prep, grid(pl.bd.size()), dim3(256), 0, st, aD, I + pl.all_off));
   for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
     if (!pl.mrg_cnt[depth]) continue;
     const int mc = pl.mcap[depth];
-    const size_t lds = (size_t)mc * (4 * sizeof(double) + sizeof(int)) + 16;
-    NRG_TIMED(ctx, st, "k_dc_merge", hipLaunchKernelGGL(k_dc_merge, grid(pl.mrg_cnt[depth]), dim3(NT_DC), lds, st, aD,
-                       (const int2 *)(I + pl.mrg_off[depth]), depth, mc));
+    const size_t lds2 = 2 * sizeof(double) * (size_t)mc + 16, lds3 = 3 * sizeof(double) * (size_t)mc + 16;
+    const int2 *mi = (const int2 *)(I + pl.mrg_off[depth]);
+    const int2 *ch = (const int2 *)(I + pl.ch_off[depth]);
+    const int2 *fc = (const int2 *)(I + pl.fc_off[depth]);
+    NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate, grid(pl.mrg_cnt[depth]), dim3(NT_DEF), lds2, st, aD,
+                                                          mi, depth, L.mst));
+    NRG_TIMED(ctx, st, "k_dc_roots", hipLaunchKernelGGL(k_dc_roots, grid(pl.rc_cnt[depth]), dim3(RCH), lds2, st, aD, mi,
+                                                        (const int2 *)(I + pl.rc_off[depth]), depth, (const MergeState *)L.mst));
+    NRG_TIMED(ctx, st, "k_dc_zhat", hipLaunchKernelGGL(k_dc_zhat, grid(pl.ch_cnt[depth]), dim3(256), lds3, st, aD, mi, ch,
+                                                       depth, (const MergeState *)L.mst));
+    NRG_TIMED(ctx, st, "k_dc_normrank", hipLaunchKernelGGL(k_dc_normrank, grid(pl.ch_cnt[depth]), dim3(256), lds3, st, aD,
+                                                           mi, ch, depth, (const MergeState *)L.mst));
+    NRG_TIMED(ctx, st, "k_dc_fill", hipLaunchKernelGGL(k_dc_fill, grid(pl.fc_cnt[depth]), dim3(256), lds2, st, aD, mi, fc,
+                                                       depth, (const MergeState *)L.mst));
     if (pl.gemm_cnt[depth])
-      NRG_TIMED(ctx, st, "k_gemm_nn", hipLaunchKernelGGL(k_gemm_nn, grid(pl.gemm_cnt[depth]), dim3(256), 0, st, L.dP,
+      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.gemm_cnt[depth]), dim3(256), 0, st, L.dG,
                          (const int2 *)(I + pl.gemm_off[depth])));
   }
   NRG_HIP(hipGetLastError());
@@ -1076,10 +1590,27 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     const size_t cnt = (size_t)pl.sumn2;
     NRG_TIMED(ctx, st, "k_convert", hipLaunchKernelGGL(k_convert<T>, dim3(2048), dim3(256), 0, st, (const double *)aD.Z0, aT.Zt, cnt));
   }
-  if (pl.tf_cnt)
-    NRG_TIMED(ctx, st, "k_bt_tfactor", hipLaunchKernelGGL(k_bt_tfactor<T>, grid(pl.tf_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.tf_off)));
-  if (pl.bt_cnt)
-    NRG_TIMED(ctx, st, "k_bt_apply", hipLaunchKernelGGL(k_bt_apply<T>, grid(pl.bt_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.bt_off)));
+  if constexpr (sizeof(T) == 8) {
+    if (pl.ym_cnt)
+      NRG_TIMED(ctx, st, "k_bt_ymat", hipLaunchKernelGGL(k_bt_ymat, grid(pl.ym_cnt), dim3(256), 0, st, aD, (const int2 *)(I + pl.ym_off)));
+    if (pl.g_cnt)
+      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.g_cnt), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.g_items)));
+    if (pl.tb_cnt)
+      NRG_TIMED(ctx, st, "k_bt_tbig", hipLaunchKernelGGL(k_bt_tbig, grid(pl.tb_cnt), dim3(256), 0, st, aD, (const int2 *)(I + pl.tb_off), L.toff));
+    if (pl.yh_cnt)
+      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.yh_cnt), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.yh_items)));
+    for (int step = 0; step < pl.Smax; step++) {
+      if (pl.w_cnt[step])
+        NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.w_cnt[step]), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.w_items[step])));
+      if (pl.u_cnt[step])
+        NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.u_cnt[step]), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.u_items[step])));
+    }
+  } else {
+    if (pl.tf_cnt)
+      NRG_TIMED(ctx, st, "k_bt_tfactor", hipLaunchKernelGGL(k_bt_tfactor<T>, grid(pl.tf_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.tf_off)));
+    if (pl.bt_cnt)
+      NRG_TIMED(ctx, st, "k_bt_apply", hipLaunchKernelGGL(k_bt_apply<T>, grid(pl.bt_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.bt_off)));
+  }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 3], st));
   return NRG_OK;
@@ -1114,42 +1645,57 @@ static int finish_flags(nrg_ctx *ctx, int nblk, int *info, hipStream_t st) {
 int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                   double *w, int *info, hipStream_t st) {
   SyevPlan pl;
-  build_plan(pl, nblk, n, off_H, off_w, true);
+  build_plan(pl, nblk, n, off_H, off_w, true, 8);
   Carver cv;
   const size_t oZ0 = cv.take<double>(pl.sumn2), oZ1 = cv.take<double>(pl.sumn2), oS = cv.take<double>(pl.sumn2);
   const size_t oWp = cv.take<double>(pl.sumWp), oTau = cv.take<double>(pl.sumn), oTf = cv.take<double>(pl.sumT);
   const size_t od = cv.take<double>(pl.sumn), oe = cv.take<double>(pl.sumn), ol = cv.take<double>(pl.sumn);
-  const size_t osc = cv.take<double>(nblk), odscr = cv.take<double>(8 * pl.sumn), oiscr = cv.take<int>(8 * pl.sumn);
+  const size_t osc = cv.take<double>(nblk), odscr = cv.take<double>(DS_COUNT * pl.sumn), oiscr = cv.take<int>(IS_COUNT * pl.sumn);
+  const size_t ovb = cv.take<double>(pl.sumn), oYp = cv.take<double>(pl.sumYp);
+  const size_t omst = cv.take<MergeState>(pl.mrg_max + 1), oamax = cv.take<double>(nblk);
+  const size_t oGb = cv.take<double>(pl.sumTb + 1), oTb = cv.take<double>(pl.sumTb + 1), oW2 = cv.take<double>(pl.sumW2);
   if (int r = ctx->ws.ensure(cv.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
-  build_gemms(pl, Z0, Z1, S);
-  // descriptor upload: [BlkDesc][GemmNN][ints]
-  const size_t bBD = sizeof(BlkDesc) * pl.bd.size(), bP = sizeof(GemmNN) * pl.probs.size(), bI = sizeof(int) * pl.ibuf.size();
-  const size_t oBD = 0, oP = (bBD + 255) & ~(size_t)255, oI = oP + ((bP + 255) & ~(size_t)255);
+  build_gemms(pl, Z0, Z1, S, ctx->ws.as<double>(oGb), ctx->ws.as<double>(oTb), ctx->ws.as<double>(oW2), H,
+              ctx->ws.as<double>(oWp));
+  // descriptor upload: [BlkDesc][GemmDesc][toff][ints]
+  const size_t bBD = sizeof(BlkDesc) * pl.bd.size(), bP = sizeof(GemmDesc) * pl.gd.size(), bI = sizeof(int) * pl.ibuf.size();
+  const size_t bT = sizeof(int64_t) * pl.toff.size();
+  const size_t oBD = 0, oP = (bBD + 255) & ~(size_t)255, oT = oP + ((bP + 255) & ~(size_t)255);
+  const size_t oI = oT + ((bT + 255) & ~(size_t)255);
   std::vector<char> blob(oI + bI);
   memcpy(blob.data() + oBD, pl.bd.data(), bBD);
-  if (bP) memcpy(blob.data() + oP, pl.probs.data(), bP);
+  if (bP) memcpy(blob.data() + oP, pl.gd.data(), bP);
+  if (bT) memcpy(blob.data() + oT, pl.toff.data(), bT);
   memcpy(blob.data() + oI, pl.ibuf.data(), bI);
   if (int r = ctx->flags.ensure(sizeof(int) * nblk)) return r;
   if (int r = ctx->hflags.ensure(sizeof(int) * nblk)) return r;
   NRG_HIP(hipEventRecord(ctx->ev[0], st));
   if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
   NRG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * nblk, st));
+  NRG_HIP(hipMemsetAsync(ctx->ws.as<double>(oamax), 0, sizeof(double) * nblk, st));
   SyevArgs<double> a;
   a.bd = (const BlkDesc *)(ctx->desc.as<char>(oBD));
   a.A = H;
   a.Wp = ctx->ws.as<double>(oWp); a.tau = ctx->ws.as<double>(oTau); a.Tf = ctx->ws.as<double>(oTf);
+  a.vb = ctx->ws.as<double>(ovb); a.Yp = ctx->ws.as<double>(oYp);
   a.Zt = Z0; a.Z0 = Z0; a.Z1 = Z1; a.S = S;
   a.d = ctx->ws.as<double>(od); a.e = ctx->ws.as<double>(oe); a.lam = ctx->ws.as<double>(ol);
   a.scale = ctx->ws.as<double>(osc);
   a.dscr = ctx->ws.as<double>(odscr); a.iscr = ctx->ws.as<int>(oiscr); a.Ntot = pl.sumn;
   a.flags = (int *)ctx->flags.p;
+  a.amax = ctx->ws.as<double>(oamax);
   a.H = H; a.w = w;
   SyevLaunch L;
   L.dI = (const int *)(ctx->desc.as<char>(oI));
-  L.dP = (const GemmNN *)(ctx->desc.as<char>(oP));
+  L.dG = (const GemmDesc *)(ctx->desc.as<char>(oP));
+  L.toff = (const int64_t *)(ctx->desc.as<char>(oT));
+  a.Gb = ctx->ws.as<double>(oGb); a.Tb = ctx->ws.as<double>(oTb);
+  L.mst = ctx->ws.as<MergeState>(omst);
   if (int r = launch_core<double>(ctx, pl, a, a, L, st, true, 0)) return r;
   NRG_TIMED(ctx, st, "k_finish", hipLaunchKernelGGL(k_finish<double>, dim3((unsigned)nblk), dim3(256), 0, st, a, L.dI + pl.all_off));
+  NRG_TIMED(ctx, st, "k_copy_out", hipLaunchKernelGGL(k_copy_out<double>, dim3((unsigned)pl.cp_cnt), dim3(256), 0, st, a,
+                                                      (const int2 *)(L.dI + pl.cp_off)));
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[4], st));
   const int r = finish_flags(ctx, nblk, info, st);

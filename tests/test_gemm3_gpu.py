@@ -1,0 +1,97 @@
+"""Parity of nrg_gemm3_grouped (FP64) with the oracle's dgemm-based products (c++/numerics_ublas.hpp:108-140)
+and with long-double brute force. Tolerance: normwise 1e-13 relative to the sum of |terms|."""
+import numpy as np
+import pytest
+
+import oracle
+from nrgljubljana_amd import NRG_GEMM_FP64, NrgError
+from tests.gemm3_cases import Mat, exact, random_table, term, to_struct
+
+pytestmark = pytest.mark.gpu
+
+
+def upload(mats):
+    import torch
+    for M in mats:
+        M.dev = torch.from_numpy(M.buf.copy()).cuda()
+
+
+def all_mats(terms, outs):
+    seen, res = set(), []
+    for M in list(outs) + [t[k] for t in terms for k in ("A", "B", "O") if t[k] is not None]:
+        if id(M) not in seen:
+            seen.add(id(M))
+            res.append(M)
+    return res
+
+
+def run_both(ctx, outs, terms, precision=NRG_GEMM_FP64):
+    import torch
+    upload(all_mats(terms, outs))
+    ctx.gemm3_grouped([to_struct(t, dev=True) for t in terms], precision)
+    torch.cuda.synchronize()
+    gpu = [C.dev.cpu().numpy().reshape(C.rows, C.ld)[:, :C.cols].copy() for C in outs]
+    ref_ld = [exact(terms, C).astype(np.float64) for C in outs]
+    oracle.gemm3([to_struct(t, dev=False) for t in terms])
+    ref = [C.view().copy() for C in outs]
+    return gpu, ref, ref_ld
+
+
+def scale_of(terms, C):
+    s = 1.0
+    for t in terms:
+        if t["C"] is C:
+            s += abs(t["f"]) * float(np.abs(t["A"].view()).sum() + 1) * float(np.abs(t["B"].view()).max() + 1) * \
+                 (float(np.abs(t["O"].view()).max() + 1) if t["O"] is not None else 1.0)
+    return s
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
+def test_random_tables(ctx, seed):
+    rng = np.random.default_rng(seed)
+    outs, terms = random_table(rng, nout=8, maxdim=150)
+    gpu, ref, ref_ld = run_both(ctx, outs, terms)
+    for C, g, r, e in zip(outs, gpu, ref, ref_ld):
+        s = float(np.max(np.abs(e))) + 1.0
+        assert np.max(np.abs(g - e)) <= 1e-13 * s
+        assert np.max(np.abs(g - r)) <= 1e-13 * s
+
+
+def test_recalc_shapes_large(ctx):
+    # transform rows as recalc_general builds them: U blocks are column slices of eigenvector
+    # matrices (ld = dim), several table rows sum into one <I1||O||Ip>
+    rng = np.random.default_rng(21)
+    outs, terms = [], []
+    for dim1, dimp, r1, rp in [(700, 650, 300, 280), (1200, 900, 500, 420), (64, 2000, 30, 700)]:
+        C = Mat(rng, dim1, dimp, fill=False)
+        outs.append(C)
+        for _ in range(3):
+            U1 = Mat(rng, dim1, r1, pad=17)
+            Up = Mat(rng, dimp, rp, pad=5)
+            O = Mat(rng, r1, rp)
+            terms.append(term("transform", C, U1, Up, O, f=float(rng.uniform(-1, 1))))
+    gpu, ref, ref_ld = run_both(ctx, outs, terms)
+    for g, r in zip(gpu, ref):
+        assert np.max(np.abs(g - r)) <= 1e-13 * (np.max(np.abs(r)) + 1.0) * 10
+
+
+def test_rotate_identity(ctx):
+    rng = np.random.default_rng(22)
+    n = 300
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    U = Mat(rng, n, n)
+    U.view()[:] = q
+    O = Mat(rng, n, n)
+    O.view()[:] = np.eye(n)
+    C = Mat(rng, n, n, fill=False)
+    gpu, _, _ = run_both(ctx, [C], [term("rotate", C, U, U, O)])
+    assert np.max(np.abs(gpu[0] - np.eye(n))) < 1e-13
+
+
+def test_bad_terms_are_rejected(ctx):
+    rng = np.random.default_rng(23)
+    C = Mat(rng, 4, 4)
+    t = to_struct(term("product", C, Mat(rng, 4, 3), Mat(rng, 4, 3)), dev=False)
+    t.l = 5   # O NULL requires k == l
+    with pytest.raises(NrgError):
+        ctx.gemm3_grouped([t])

@@ -129,6 +129,15 @@ def test_special_matrices(ctx, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_extreme_scales(ctx, mode):
+    # dsyev rescales blocks whose max-norm leaves [sqrt(safmin/eps), sqrt(bignum)] ~ [1e-146, 1e146]
+    rng = np.random.default_rng(16)
+    X = workloads.eig_block(rng, 33)[0]
+    blocks = [s * X for s in (1e-300, 1e-200, 1e-160, 1e-150, 1e-120, 1e120, 1e150, 1e160, 1e200, 1e300)]
+    check_against_oracle(ctx, blocks, mode, label="scales")
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_clustered_spectra(ctx, mode):
     rng = np.random.default_rng(13)
     blocks = [workloads.eig_block(rng, n)[0] for n in [16, 31, 64, 65, 100, 127, 128, 129, 257, 300]]
