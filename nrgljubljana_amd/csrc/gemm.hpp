@@ -9,31 +9,38 @@
 namespace nrg {
 
 // One K-part of a grouped product: scale * op(A) op(B), op(A) M x K, op(B) K x N.
-struct GemmPart {
-  const double *A, *B;
+template <typename T>
+struct GemmPartT {
+  const T *A, *B;
   int lda, ldb, tA, tB, K;
   double scale;
 };
 
-struct GemmDesc {
-  const double *A, *B, *A2, *B2;
-  double *C;
+template <typename T>
+struct GemmDescT {
+  const T *A, *B, *A2, *B2;
+  T *C;
   int M, N, K, K2;
   int lda, ldb, lda2, ldb2, ldc;
-  int tA, tB, tA2, tB2;   // 1: use the transpose
-  double alpha, beta;     // beta == 0: C is not read
-  const GemmPart *parts;  // if nparts > 0: C = alpha * sum_p scale_p op(A_p) op(B_p) + beta * C
+  int tA, tB, tA2, tB2;        // 1: use the transpose
+  double alpha, beta;          // beta == 0: C is not read
+  const GemmPartT<T> *parts;   // if nparts > 0: C = alpha * sum_p scale_p op(A_p) op(B_p) + beta * C
   int nparts;
 };
+using GemmDesc = GemmDescT<double>;
+using GemmPart = GemmPartT<double>;
+using GemmDesc32 = GemmDescT<float>;
 
 constexpr int GEMM_TILE = 64;
 
-inline int gemm_tiles(const GemmDesc &g) {
+template <typename T>
+inline int gemm_tiles(const GemmDescT<T> &g) {
   return ((g.M + GEMM_TILE - 1) / GEMM_TILE) * ((g.N + GEMM_TILE - 1) / GEMM_TILE);
 }
 
 // appends (desc index, tile) items of descriptors [d0, d1) to `items`
-inline void gemm_items(const std::vector<GemmDesc> &d, size_t d0, size_t d1, std::vector<int> &items) {
+template <typename T>
+inline void gemm_items(const std::vector<GemmDescT<T>> &d, size_t d0, size_t d1, std::vector<int> &items) {
   for (size_t i = d0; i < d1; i++) {
     const int t = gemm_tiles(d[i]);
     for (int k = 0; k < t; k++) {
@@ -44,5 +51,6 @@ inline void gemm_items(const std::vector<GemmDesc> &d, size_t d0, size_t d1, std
 }
 
 __global__ void k_gemm_f64(const GemmDesc *descs, const int2 *items);
+__global__ void k_gemm_f32(const GemmDesc32 *descs, const int2 *items);
 
 }  // namespace nrg

@@ -1,11 +1,6 @@
 // stubs.hip - entry points whose kernels are not built yet.
 #include "common.hpp"
 namespace nrg {
-int syev_mixed_run(nrg_ctx *, int, const int64_t *, const int64_t *, double *, const int64_t *, double *, int *,
-                   hipStream_t) {
-  set_error("nrg_syev_batched: mixed form not built yet");
-  return NRG_E_ARG;
-}
 int gemm3_ozaki(nrg_ctx *, const nrg_gemm3_term *, int, hipStream_t) {
   set_error("nrg_gemm3_grouped: split-precision form not built yet");
   return NRG_E_ARG;

@@ -25,12 +25,13 @@
 #include "gemm.hpp"
 #include <algorithm>
 #include <cstring>
+#include <cstdlib>
 #include <cmath>
 
 namespace nrg {
 
 constexpr int NB = 32;        // panel width of the Householder reduction
-constexpr int NT_PANEL = 512; // threads of the column-preparation kernel
+constexpr int NT_PANEL = 1024; // threads of the column-preparation kernel
 constexpr int SMALLM = 64;    // merges up to this order do their own block-diagonal product
 
 struct BlkDesc {
@@ -47,7 +48,8 @@ struct SyevArgs {
   T *Tf;           // compact-WY T factors, P x NB x NB per block
   T *vb;           // current reflector of every block, n per block
   T *Yp;           // symmetric-product tile partials, n x ceil(n/64) per block
-  double *Gb, *Tb; // FP64 back-transformation: G_s and T_s of every super-panel
+  T *Ym, *Yh;      // back-transformation: explicit reflectors Y and Yhat = Y T^T (n x n per block)
+  T *Gb, *Tb;      // back-transformation: G_s and T_s of every super-panel
   T *Zt;           // eigenvector matrix used by the back-transformation (== Z0 for FP64)
   double *Z0, *Z1, *S;
   double *d, *e, *lam, *scale;
@@ -99,8 +101,9 @@ __device__ __forceinline__ double block_sigma(double amax) {
   return (amax > 0 && isfinite(amax)) ? ldexp(1.0, -ilogb(amax)) : 1.0;
 }
 
+// dst_h: write the full matrix in place into the caller's arena (FP64 only) instead of the work matrix
 template <typename T>
-__global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items) {
+__global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items, int dst_h) {
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
   const int n = (int)bd.n;
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items) 
   while (ti * (ti + 1) / 2 > t) ti--;
   const int tj = t - ti * (ti + 1) / 2;
   const double *Hs = a.H + bd.offH;
-  T *Ad = a.A + bd.offA;
+  T *Ad = dst_h ? (T *)(a.H + bd.offH) : a.A + bd.offA;
   const double sigma = block_sigma(a.amax[it.x]);
   __shared__ T tile[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -601,13 +604,23 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
   }
   const double dorg = d[o];
   for (int it = 0; it < 100; it++) {
-    double R = 0, Rp = 0, Ra = 0;
-    for (int i = 0; i < k; i++) {
-      if (i == o || i == o2) continue;
-      const double q = 1.0 / ((d[i] - dorg) - t);
-      const double tt = rho * z[i] * z[i] * q;
-      R += tt; Rp += tt * q; Ra += fabs(tt);
+    // the two adjacent poles are masked out (their denominators -t and d2 - t are nonzero);
+    // independent divisions in flight hide their latency
+    double R = 0, Rp = 0, Ra = 0, R2 = 0, Rp2 = 0, Ra2 = 0;
+    int i = 0;
+    for (; i + 1 < k; i += 2) {
+      const double q0 = 1.0 / ((d[i] - dorg) - t), q1 = 1.0 / ((d[i + 1] - dorg) - t);
+      const double t0 = (i == o || i == o2) ? 0.0 : rho * z[i] * z[i] * q0;
+      const double t1 = (i + 1 == o || i + 1 == o2) ? 0.0 : rho * z[i + 1] * z[i + 1] * q1;
+      R += t0; Rp += t0 * q0; Ra += fabs(t0);
+      R2 += t1; Rp2 += t1 * q1; Ra2 += fabs(t1);
     }
+    if (i < k) {
+      const double q0 = 1.0 / ((d[i] - dorg) - t);
+      const double t0 = (i == o || i == o2) ? 0.0 : rho * z[i] * z[i] * q0;
+      R += t0; Rp += t0 * q0; Ra += fabs(t0);
+    }
+    R += R2; Rp += Rp2; Ra += Ra2;
     double f, err, gv, gp;
     const double pa = -za / t;
     if (o2 >= 0) {
@@ -929,38 +942,43 @@ __global__ __launch_bounds__(256) void k_dc_fill(SyevArgs<double> a, const int2 
 constexpr int KBT = 128;
 
 // Y (column j = reflector j: zeros down to row j, 1 at row j+1, v below), 64 columns per item
-__global__ __launch_bounds__(256) void k_bt_ymat(SyevArgs<double> a, const int2 *items) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_ymat(SyevArgs<T> a, const int2 *items) {
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
   const int n = (int)bd.n;
   const int j0 = it.y * 64, j1 = min(n - 1, j0 + 64);
-  const double *A = a.A + bd.offA;
-  double *Y = a.S + bd.offZ;
+  const T *A = a.A + bd.offA;
+  T *Y = a.Ym + bd.offZ;
   for (int j = j0; j < j1; j++)
     for (int r = threadIdx.x; r < n; r += 256)
-      Y[(size_t)j * n + r] = r == j + 1 ? 1.0 : (r > j + 1 ? A[(size_t)j * n + r] : 0.0);
+      Y[(size_t)j * n + r] = r == j + 1 ? (T)1 : (r > j + 1 ? A[(size_t)j * n + r] : (T)0);
 }
 
 // T_s (kb x kb, upper triangular, column-major) from G_s and the reflector scalars
-__global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<double> a, const int2 *items, const int64_t *toff) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<T> a, const int2 *items, const int64_t *toff) {
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
   const int n = (int)bd.n, sp = it.y;
   const int kb = min(KBT, (n - 1) - sp * KBT);
   const size_t off = (size_t)toff[it.x] + (size_t)sp * KBT * KBT;
-  const double *G = a.Gb + off;     // G(c, i) = G[i*kb + c]
-  double *Tg = a.Tb + off;
-  const double *tau = a.tau + bd.offV + sp * KBT;
-  __shared__ double Ts[KBT][KBT + 1];
+  const T *G = a.Gb + off;     // G(c, i) = G[i*kb + c]
+  T *Tg = a.Tb + off;
+  const T *tau = a.tau + bd.offV + sp * KBT;
+  __shared__ T Ts[KBT][KBT + 1];
+  __shared__ T gcol[KBT];
   for (int i = 0; i < kb; i++) {
-    const double ti = tau[i];
+    const T ti = tau[i];
+    for (int c = threadIdx.x; c < i; c += 256) gcol[c] = G[(size_t)i * kb + c];
+    __syncthreads();
     for (int x = threadIdx.x; x < i; x += 256) {
-      double sum = 0;
-      for (int c = x; c < i; c++) sum += Ts[x][c] * G[(size_t)i * kb + c];
+      T sum = 0;
+      for (int c = x; c < i; c++) sum += Ts[x][c] * gcol[c];
       Ts[x][i] = -ti * sum;
     }
     if (threadIdx.x == 0) Ts[i][i] = ti;
-    for (int x = i + 1 + threadIdx.x; x < kb; x += 256) Ts[x][i] = 0.0;
+    for (int x = i + 1 + threadIdx.x; x < kb; x += 256) Ts[x][i] = (T)0;
     __syncthreads();
   }
   for (int idx = threadIdx.x; idx < kb * kb; idx += 256) Tg[idx] = Ts[idx % kb][idx / kb];
@@ -1153,6 +1171,271 @@ template <typename T>
 __global__ void k_convert(const double *src, T *dst, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = (T)src[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// 6. Mixed form: Ogita-Aishima refinement in FP64 of the FP32 eigenvectors.
+//    For X with A X ~ X L:  R = I - X^T X,  S = X^T A X,  l_i = s_ii / (1 - r_ii),
+//    E_ij = (s_ij + l_j r_ij) / (l_j - l_i) if |l_i - l_j| > delta, else r_ij / 2,
+//    X <- X (I + E + K^2 / 2) with K the skew part of E (the second-order term of exp(K), which
+//    keeps the orthogonality error cubic instead of quadratic in K);
+//    delta = 2 (||S - diag(l)|| + ||A|| ||R||) (Frobenius bounds).
+//    Eigenvalues closer than delta form clusters whose Ritz problem
+//    B = S_CC + (R_CC S_CC + S_CC R_CC) / 2 is solved by Jacobi and rotates X_C.
+// ------------------------------------------------------------------------------------------
+struct RefArgs {
+  const BlkDesc *bd;
+  const double *A;      // scaled symmetric input, full storage (the H arena)
+  double *X, *Xn, *AX, *S, *R, *E;   // n x n per block at offZ
+  double *lt;           // eigenvalue estimates, n per block at offV
+  double *acc;          // per block: ||S - diag l||_F^2, ||R||_F^2, max |l|
+  int *flags;
+};
+
+// X = (double) V32, R = I
+__global__ __launch_bounds__(256) void k_ref_init(RefArgs r, const float *V32, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = r.bd[it.x];
+  const int n = (int)bd.n;
+  const size_t nn = (size_t)n * n;
+  const size_t beg = (size_t)it.y * COPY_CHUNK, end = min(nn, beg + COPY_CHUNK);
+  for (size_t i = beg + threadIdx.x; i < end; i += 256) {
+    r.X[bd.offZ + i] = (double)V32[bd.offZ + i];
+    r.R[bd.offZ + i] = (i % n == i / n) ? 1.0 : 0.0;
+  }
+}
+
+// R = I again before the next sweep's X^T X product
+__global__ __launch_bounds__(256) void k_ref_eye(RefArgs r, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = r.bd[it.x];
+  const int n = (int)bd.n;
+  const size_t nn = (size_t)n * n;
+  const size_t beg = (size_t)it.y * COPY_CHUNK, end = min(nn, beg + COPY_CHUNK);
+  for (size_t i = beg + threadIdx.x; i < end; i += 256) r.R[bd.offZ + i] = (i % n == i / n) ? 1.0 : 0.0;
+}
+
+// eigenvalue estimates l_i = s_ii / (1 - r_ii), max |l|
+__global__ __launch_bounds__(256) void k_ref_lam(RefArgs r, const int *blist) {
+  const int b = blist[blockIdx.x];
+  const BlkDesc bd = r.bd[b];
+  const int n = (int)bd.n;
+  __shared__ double red[4];
+  double mx = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const size_t di = bd.offZ + (size_t)i * n + i;
+    const double l = r.S[di] / (1.0 - r.R[di]);
+    r.lt[bd.offV + i] = l;
+    mx = fmax(mx, fabs(l));
+  }
+  mx = block_max<256>(mx, red);
+  if (threadIdx.x == 0) r.acc[3 * b + 2] = mx;
+}
+
+// Frobenius norms ||S - diag l||^2 and ||R||^2, 16 columns per item
+__global__ __launch_bounds__(256) void k_ref_norms(RefArgs r, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = r.bd[it.x];
+  const int n = (int)bd.n;
+  const int c0 = it.y * 16, c1 = min(n, c0 + 16);
+  __shared__ double red[4];
+  double s2 = 0, r2 = 0;
+  for (int c = c0; c < c1; c++)
+    for (int i = threadIdx.x; i < n; i += 256) {
+      const size_t k = bd.offZ + (size_t)c * n + i;
+      const double sv = r.S[k] - (i == c ? r.lt[bd.offV + i] : 0.0);
+      s2 += sv * sv;
+      r2 += r.R[k] * r.R[k];
+    }
+  s2 = block_sum<256>(s2, red);
+  r2 = block_sum<256>(r2, red);
+  if (threadIdx.x == 0) {
+    atomicAdd(&r.acc[3 * it.x], s2);
+    atomicAdd(&r.acc[3 * it.x + 1], r2);
+  }
+}
+
+// Cluster threshold. Besides the Ogita-Aishima bound, pairs closer than 1e-6 ||A|| are always
+// treated as a cluster: there the correction (s_ij + l_j r_ij) / (l_j - l_i) would be dominated by
+// the rounding noise of s_ij (~ sqrt(n) eps ||A||), and its skew-symmetric part costs
+// orthogonality quadratically; the cluster's Ritz step is exactly orthogonal instead.
+constexpr double CLUSTER_FLOOR = 1e-6;
+__device__ __forceinline__ double ref_delta(const RefArgs &r, int b) {
+  const double anorm = r.acc[3 * b + 2];
+  return fmax(2.0 * (sqrt(r.acc[3 * b]) + anorm * sqrt(r.acc[3 * b + 1])), CLUSTER_FLOOR * anorm);
+}
+
+// E' = I + E, 16 columns per item
+__global__ __launch_bounds__(256) void k_ref_E(RefArgs r, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = r.bd[it.x];
+  const int n = (int)bd.n;
+  const int c0 = it.y * 16, c1 = min(n, c0 + 16);
+  const double delta = ref_delta(r, it.x);
+  const double *lt = r.lt + bd.offV;
+  // S and R enter through their symmetric parts: the computed X^T (A X) and X^T X are symmetric
+  // only up to rounding, and the antisymmetric rest, divided by l_j - l_i, would leave
+  // E + E^T != R and stall the orthogonality at (s_ij - s_ji) / gap.
+  for (int j = c0; j < c1; j++) {
+    const double lj = lt[j];
+    for (int i = threadIdx.x; i < n; i += 256) {
+      const size_t k = bd.offZ + (size_t)j * n + i, kt = bd.offZ + (size_t)i * n + j;
+      const double rij = 0.5 * (r.R[k] + r.R[kt]);
+      double e, kij = 0.0;
+      if (i == j) {
+        e = 1.0 + 0.5 * rij;
+      } else {
+        const double li = lt[i];
+        if (fabs(li - lj) > delta) {
+          const double sij = 0.5 * (r.S[k] + r.S[kt]);
+          const double eij = (sij + lj * rij) / (lj - li), eji = (sij + li * rij) / (li - lj);
+          e = eij;
+          kij = 0.5 * (eij - eji);
+        } else {
+          e = 0.5 * rij;
+        }
+      }
+      r.E[k] = e;
+      r.AX[k] = kij;   // skew part K of the correction (A X is no longer needed)
+    }
+  }
+}
+
+constexpr int CMAX = 64;   // largest cluster solved as one Ritz problem
+
+// clusters of eigenvalue estimates closer than delta: Ritz problem by cyclic Jacobi, rotate Xn_C
+__global__ __launch_bounds__(256) void k_ref_clusters(RefArgs r, const int *blist) {
+  const int b = blist[blockIdx.x];
+  const BlkDesc bd = r.bd[b];
+  const int n = (int)bd.n;
+  const double delta = ref_delta(r, b);
+  double *lt = r.lt + bd.offV;
+  const double *S = r.S + bd.offZ, *R = r.R + bd.offZ;
+  double *X = r.Xn + bd.offZ;
+  __shared__ double B[CMAX][CMAX + 1], Q[CMAX][CMAX + 1], Bt[CMAX][CMAX + 1];
+  __shared__ double ev[CMAX];
+  __shared__ int perm[CMAX];
+  __shared__ double cs[2];
+  __shared__ double lts[NMAX];
+  for (int i = threadIdx.x; i < n; i += 256) lts[i] = lt[i];
+  __syncthreads();
+  int start = 0;
+  while (start < n) {
+    int end = start + 1;
+    while (end < n && end - start < CMAX && fabs(lts[end] - lts[end - 1]) <= delta) end++;
+    const int c = end - start;
+    if (c >= 2) {
+      // B = S_CC + (R_CC S_CC + S_CC R_CC) / 2
+      for (int idx = threadIdx.x; idx < c * c; idx += 256) {
+        const int i = idx % c, j = idx / c;
+        double v = S[(size_t)(start + j) * n + start + i];
+        double rs = 0;
+        for (int k = 0; k < c; k++)
+          rs += R[(size_t)(start + k) * n + start + i] * S[(size_t)(start + j) * n + start + k] +
+                S[(size_t)(start + k) * n + start + i] * R[(size_t)(start + j) * n + start + k];
+        B[i][j] = v + 0.5 * rs;
+        Q[i][j] = i == j ? 1.0 : 0.0;
+      }
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < c * c; idx += 256) {   // symmetrise
+        const int i = idx % c, j = idx / c;
+        Bt[i][j] = 0.5 * (B[i][j] + B[j][i]);
+      }
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < c * c; idx += 256) B[idx % c][idx / c] = Bt[idx % c][idx / c];
+      __syncthreads();
+      for (int sweep = 0; sweep < 30; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < c; i++)
+          for (int j = 0; j < c; j++) {
+            tot += B[i][j] * B[i][j];
+            if (i != j) off += B[i][j] * B[i][j];
+          }
+        if (off <= 1e-34 * tot || off == 0.0) break;
+        for (int p = 0; p < c - 1; p++)
+          for (int q = p + 1; q < c; q++) {
+            if (threadIdx.x == 0) {
+              const double apq = B[p][q];
+              double cc = 1.0, ss = 0.0;
+              if (apq != 0.0) {
+                const double th = (B[q][q] - B[p][p]) / (2.0 * apq);
+                const double t = copysign(1.0, th) / (fabs(th) + sqrt(1.0 + th * th));
+                cc = 1.0 / sqrt(1.0 + t * t);
+                ss = t * cc;
+              }
+              cs[0] = cc;
+              cs[1] = ss;
+            }
+            __syncthreads();
+            const double cc = cs[0], ss = cs[1];
+            if (ss != 0.0) {
+              const int k = threadIdx.x;
+              if (k < c) {   // columns p, q
+                const double bkp = B[k][p], bkq = B[k][q];
+                B[k][p] = cc * bkp - ss * bkq;
+                B[k][q] = ss * bkp + cc * bkq;
+                const double qkp = Q[k][p], qkq = Q[k][q];
+                Q[k][p] = cc * qkp - ss * qkq;
+                Q[k][q] = ss * qkp + cc * qkq;
+              }
+              __syncthreads();
+              if (k < c) {   // rows p, q
+                const double bpk = B[p][k], bqk = B[q][k];
+                B[p][k] = cc * bpk - ss * bqk;
+                B[q][k] = ss * bpk + cc * bqk;
+              }
+            }
+            __syncthreads();
+          }
+      }
+      // ascending order of the Ritz values
+      if (threadIdx.x == 0) {
+        for (int i = 0; i < c; i++) { perm[i] = i; ev[i] = B[i][i]; }
+        for (int i = 1; i < c; i++) {
+          int j = i;
+          while (j > 0 && ev[perm[j - 1]] > ev[perm[j]]) { const int t = perm[j]; perm[j] = perm[j - 1]; perm[j - 1] = t; j--; }
+        }
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < c; i += 256) lt[start + i] = ev[perm[i]];   // lts keeps the scan order
+      // X_C <- X_C Q(:, perm)
+      for (int row = threadIdx.x; row < n; row += 256) {
+        double xr[CMAX];
+        for (int k = 0; k < c; k++) xr[k] = X[(size_t)(start + k) * n + row];
+        for (int j = 0; j < c; j++) {
+          double v = 0;
+          for (int k = 0; k < c; k++) v += xr[k] * Q[k][perm[j]];
+          X[(size_t)(start + j) * n + row] = v;
+        }
+      }
+      __syncthreads();
+    }
+    start = end;
+  }
+}
+
+// results of the mixed form: w = l / sigma, eigenvectors X -> H
+__global__ __launch_bounds__(256) void k_ref_finish(RefArgs r, double *H, double *w, const double *amax,
+                                                    const int *blist) {
+  const int b = blist[blockIdx.x];
+  const BlkDesc bd = r.bd[b];
+  const int n = (int)bd.n;
+  const double sg = block_sigma(amax[b]);
+  int bad = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double v = r.lt[bd.offV + i] / sg;
+    w[bd.offw + i] = v;
+    bad |= !isfinite(v);
+  }
+  if (bad) r.flags[b] = 2;
+}
+
+__global__ __launch_bounds__(256) void k_ref_copy(RefArgs r, double *H, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const BlkDesc bd = r.bd[it.x];
+  const size_t nn = (size_t)bd.n * bd.n;
+  const size_t beg = (size_t)it.y * COPY_CHUNK, end = min(nn, beg + COPY_CHUNK);
+  for (size_t i = beg + threadIdx.x; i < end; i += 256) H[bd.offH + i] = r.X[bd.offZ + i];
 }
 
 // ==========================================================================================
@@ -1380,32 +1663,10 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   pl.cp_cnt = (I.size() - pl.cp_off) / 2;
 }
 
-// Products (pointers into the arenas): the large D&C merges at each depth and the FP64
-// back-transformation. Appends the work items to pl.ibuf.
-static void build_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S, double *Gb, double *Tb, double *W2,
-                        double *Abase, double *Wpbase) {
+// Products of the D&C merges at each depth (FP64, pointers into the arenas); appends items to pl.ibuf.
+static void build_dc_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S) {
   pl.gd.clear();
   std::vector<int> &I = pl.ibuf;
-  // trailing updates A22 -= V W^T + W V^T after every panel that is not a block's last
-  pl.s2_items.assign(pl.Pmax, 0); pl.s2_cnt.assign(pl.Pmax, 0);
-  for (int p = 0; p < pl.Pmax; p++) {
-    const size_t d0 = pl.gd.size();
-    for (const BlkDesc &d : pl.bd) {
-      if (d.P <= p + 1) continue;
-      const int n = (int)d.n, i0 = p * NB, mt = n - i0 - NB;
-      GemmDesc g{};
-      double *Ab = Abase + d.offA;
-      const double *V = Ab + (size_t)i0 * n + (i0 + NB), *Wt = Wpbase + d.offWp + NB;
-      g.A = V; g.lda = n; g.tA = 0; g.B = Wt; g.ldb = n; g.tB = 1; g.K = NB;
-      g.A2 = Wt; g.lda2 = n; g.tA2 = 0; g.B2 = V; g.ldb2 = n; g.tB2 = 1; g.K2 = NB;
-      g.C = Ab + (size_t)(i0 + NB) * n + (i0 + NB); g.ldc = n;
-      g.M = mt; g.N = mt; g.alpha = -1.0; g.beta = 1.0;
-      pl.gd.push_back(g);
-    }
-    pl.s2_items[p] = I.size();
-    gemm_items(pl.gd, d0, pl.gd.size(), I);
-    pl.s2_cnt[p] = (I.size() - pl.s2_items[p]) / 2;
-  }
   for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
     const size_t d0 = pl.gd.size();
     const int *it = I.data() + pl.mrg_off[depth];
@@ -1436,76 +1697,105 @@ static void build_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S, double 
     gemm_items(pl.gd, d0, pl.gd.size(), I);
     pl.gemm_cnt[depth] = (I.size() - pl.gemm_off[depth]) / 2;
   }
+}
+
+// Products of the reduction (trailing rank-2k updates) and of the back-transformation in the
+// work precision T; appends items to pl.ibuf.
+template <typename T>
+static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *Abase, T *Wpbase, T *Ym, T *Yh, T *Z,
+                               T *Gb, T *Tb, T *W2) {
+  gd.clear();
+  std::vector<int> &I = pl.ibuf;
+  // trailing updates A22 -= V W^T + W V^T after every panel that is not a block's last
+  pl.s2_items.assign(pl.Pmax, 0); pl.s2_cnt.assign(pl.Pmax, 0);
+  for (int p = 0; p < pl.Pmax; p++) {
+    const size_t d0 = gd.size();
+    for (const BlkDesc &d : pl.bd) {
+      if (d.P <= p + 1) continue;
+      const int n = (int)d.n, i0 = p * NB, mt = n - i0 - NB;
+      GemmDescT<T> g{};
+      T *Ab = Abase + d.offA;
+      const T *V = Ab + (size_t)i0 * n + (i0 + NB), *Wt = Wpbase + d.offWp + NB;
+      g.A = V; g.lda = n; g.tA = 0; g.B = Wt; g.ldb = n; g.tB = 1; g.K = NB;
+      g.A2 = Wt; g.lda2 = n; g.tA2 = 0; g.B2 = V; g.ldb2 = n; g.tB2 = 1; g.K2 = NB;
+      g.C = Ab + (size_t)(i0 + NB) * n + (i0 + NB); g.ldc = n;
+      g.M = mt; g.N = mt; g.alpha = -1.0; g.beta = 1.0;
+      gd.push_back(g);
+    }
+    pl.s2_items[p] = I.size();
+    gemm_items(gd, d0, gd.size(), I);
+    pl.s2_cnt[p] = (I.size() - pl.s2_items[p]) / 2;
+  }
   // back-transformation products
   const int nblk = (int)pl.bd.size();
   auto nsp = [&](int b) { const int n = (int)pl.bd[b].n; return n >= 3 ? (n - 1 + KBT - 1) / KBT : 0; };
-  size_t d0 = pl.gd.size();
+  size_t d0 = gd.size();
   for (int b = 0; b < nblk; b++) {
     const BlkDesc &d = pl.bd[b];
     const int n = (int)d.n;
     for (int sp = 0; sp < nsp(b); sp++) {
       const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
-      GemmDesc g{};
-      g.A = S + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
+      GemmDescT<T> g{};
+      g.A = Ym + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
       g.B = g.A; g.ldb = n; g.tB = 0;
       g.C = Gb + pl.toff[b] + (size_t)sp * KBT * KBT; g.ldc = kb;
       g.M = kb; g.N = kb; g.K = n - r0; g.alpha = 1.0; g.beta = 0.0;
-      pl.gd.push_back(g);
+      gd.push_back(g);
     }
   }
   pl.g_items = I.size();
-  gemm_items(pl.gd, d0, pl.gd.size(), I);
+  gemm_items(gd, d0, gd.size(), I);
   pl.g_cnt = (I.size() - pl.g_items) / 2;
-  d0 = pl.gd.size();
+  d0 = gd.size();
   for (int b = 0; b < nblk; b++) {
     const BlkDesc &d = pl.bd[b];
     const int n = (int)d.n;
     for (int sp = 0; sp < nsp(b); sp++) {
       const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
-      GemmDesc g{};
-      g.A = S + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
+      GemmDescT<T> g{};
+      g.A = Ym + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
       g.B = Tb + pl.toff[b] + (size_t)sp * KBT * KBT; g.ldb = kb; g.tB = 1;
-      g.C = Z1 + d.offZ + (size_t)sp * KBT * n + r0; g.ldc = n;
+      g.C = Yh + d.offZ + (size_t)sp * KBT * n + r0; g.ldc = n;
       g.M = n - r0; g.N = kb; g.K = kb; g.alpha = 1.0; g.beta = 0.0;
-      pl.gd.push_back(g);
+      gd.push_back(g);
     }
   }
   pl.yh_items = I.size();
-  gemm_items(pl.gd, d0, pl.gd.size(), I);
+  gemm_items(gd, d0, gd.size(), I);
   pl.yh_cnt = (I.size() - pl.yh_items) / 2;
   pl.w_items.assign(pl.Smax, 0); pl.w_cnt.assign(pl.Smax, 0);
   pl.u_items.assign(pl.Smax, 0); pl.u_cnt.assign(pl.Smax, 0);
   for (int step = 0; step < pl.Smax; step++) {
     for (int phase = 0; phase < 2; phase++) {
-      d0 = pl.gd.size();
+      d0 = gd.size();
       int64_t w2o = 0;
       for (int b = 0; b < nblk; b++) {
         const BlkDesc &d = pl.bd[b];
         const int n = (int)d.n;
         const int S_b = nsp(b);
-        double *W2b = W2 + w2o;
+        T *W2b = W2 + w2o;
         w2o += (int64_t)KBT * n;
         if (S_b <= step) continue;
         const int sp = S_b - 1 - step;
         const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
-        GemmDesc g{};
+        GemmDescT<T> g{};
         if (phase == 0) {   // W = Yhat_s^T Z(r0:, :)
-          g.A = Z1 + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
-          g.B = Z0 + d.offZ + r0; g.ldb = n; g.tB = 0;
+          g.A = Yh + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
+          g.B = Z + d.offZ + r0; g.ldb = n; g.tB = 0;
           g.C = W2b; g.ldc = kb;
           g.M = kb; g.N = n; g.K = n - r0; g.alpha = 1.0; g.beta = 0.0;
         } else {            // Z(r0:, :) -= Y_s W
-          g.A = S + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
+          g.A = Ym + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
           g.B = W2b; g.ldb = kb; g.tB = 0;
-          g.C = Z0 + d.offZ + r0; g.ldc = n;
+          g.C = Z + d.offZ + r0; g.ldc = n;
           g.M = n - r0; g.N = n; g.K = kb; g.alpha = -1.0; g.beta = 1.0;
         }
-        pl.gd.push_back(g);
+        gd.push_back(g);
       }
       size_t &io = phase == 0 ? pl.w_items[step] : pl.u_items[step];
       size_t &ic = phase == 0 ? pl.w_cnt[step] : pl.u_cnt[step];
       io = I.size();
-      gemm_items(pl.gd, d0, pl.gd.size(), I);
+      gemm_items(gd, d0, gd.size(), I);
       ic = (I.size() - io) / 2;
     }
   }
@@ -1513,10 +1803,22 @@ static void build_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S, double 
 
 struct SyevLaunch {
   const int *dI;      // device copy of ibuf
-  const GemmDesc *dG; // device copy of the product descriptors
+  const GemmDesc *dG;   // device copy of the FP64 product descriptors (D&C)
+  const void *dGT;      // device copy of the reduction/back-transformation descriptors (work precision)
   const int64_t *toff; // device copy of the per-block super-panel offsets
   MergeState *mst;    // per-merge state of the current depth
 };
+
+template <typename T>
+static void launch_gemm(nrg_ctx *ctx, hipStream_t st, const void *descs, const int *items, size_t count) {
+  if (!count) return;
+  if constexpr (sizeof(T) == 8)
+    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)count), dim3(256), 0, st,
+                                                        (const GemmDesc *)descs, (const int2 *)items));
+  else
+    NRG_TIMED(ctx, st, "k_gemm_f32", hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)count), dim3(256), 0, st,
+                                                        (const GemmDesc32 *)descs, (const int2 *)items));
+}
 
 // Launch sequence of the reduction + D&C + back-transformation for a typed work matrix.
 template <typename T>
@@ -1528,7 +1830,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   if (pl.symm_cnt)
     NRG_TIMED(ctx, st, "k_absmax", hipLaunchKernelGGL(k_absmax<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off)));
   if (pl.symm_cnt)
-    NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off)));
+    NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off), 0));
   // 2. reduction
   size_t ci = 0;
   for (int p = 0; p < pl.Pmax; p++) {
@@ -1541,12 +1843,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
         NRG_TIMED(ctx, st, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, st, aT,
                                                             I + c.symv_off, I + c.tpre_off, (int)c.nact, p, c.jl));
     }
-    if constexpr (sizeof(T) == 8) {
-      if (pl.s2_cnt[p])
-        NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.s2_cnt[p]), dim3(256), 0, st, L.dG,
-                                                            (const int2 *)(I + pl.s2_items[p])));
-    } else if (pl.upd_cnt[p])
-      NRG_TIMED(ctx, st, "k_syr2k", hipLaunchKernelGGL(k_syr2k<T>, grid(pl.upd_cnt[p]), dim3(256), 0, st, aT, (const int2 *)(I + pl.upd_off[p]), p));
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.s2_items[p], pl.s2_cnt[p]);
   }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 1], st));
@@ -1590,26 +1887,15 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     const size_t cnt = (size_t)pl.sumn2;
     NRG_TIMED(ctx, st, "k_convert", hipLaunchKernelGGL(k_convert<T>, dim3(2048), dim3(256), 0, st, (const double *)aD.Z0, aT.Zt, cnt));
   }
-  if constexpr (sizeof(T) == 8) {
-    if (pl.ym_cnt)
-      NRG_TIMED(ctx, st, "k_bt_ymat", hipLaunchKernelGGL(k_bt_ymat, grid(pl.ym_cnt), dim3(256), 0, st, aD, (const int2 *)(I + pl.ym_off)));
-    if (pl.g_cnt)
-      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.g_cnt), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.g_items)));
-    if (pl.tb_cnt)
-      NRG_TIMED(ctx, st, "k_bt_tbig", hipLaunchKernelGGL(k_bt_tbig, grid(pl.tb_cnt), dim3(256), 0, st, aD, (const int2 *)(I + pl.tb_off), L.toff));
-    if (pl.yh_cnt)
-      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.yh_cnt), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.yh_items)));
-    for (int step = 0; step < pl.Smax; step++) {
-      if (pl.w_cnt[step])
-        NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.w_cnt[step]), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.w_items[step])));
-      if (pl.u_cnt[step])
-        NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.u_cnt[step]), dim3(256), 0, st, L.dG, (const int2 *)(I + pl.u_items[step])));
-    }
-  } else {
-    if (pl.tf_cnt)
-      NRG_TIMED(ctx, st, "k_bt_tfactor", hipLaunchKernelGGL(k_bt_tfactor<T>, grid(pl.tf_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.tf_off)));
-    if (pl.bt_cnt)
-      NRG_TIMED(ctx, st, "k_bt_apply", hipLaunchKernelGGL(k_bt_apply<T>, grid(pl.bt_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.bt_off)));
+  if (pl.ym_cnt)
+    NRG_TIMED(ctx, st, "k_bt_ymat", hipLaunchKernelGGL(k_bt_ymat<T>, grid(pl.ym_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.ym_off)));
+  launch_gemm<T>(ctx, st, L.dGT, I + pl.g_items, pl.g_cnt);
+  if (pl.tb_cnt)
+    NRG_TIMED(ctx, st, "k_bt_tbig", hipLaunchKernelGGL(k_bt_tbig<T>, grid(pl.tb_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.tb_off), L.toff));
+  launch_gemm<T>(ctx, st, L.dGT, I + pl.yh_items, pl.yh_cnt);
+  for (int step = 0; step < pl.Smax; step++) {
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.w_items[step], pl.w_cnt[step]);
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.u_items[step], pl.u_cnt[step]);
   }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 3], st));
@@ -1656,16 +1942,20 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   const size_t oGb = cv.take<double>(pl.sumTb + 1), oTb = cv.take<double>(pl.sumTb + 1), oW2 = cv.take<double>(pl.sumW2);
   if (int r = ctx->ws.ensure(cv.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
-  build_gemms(pl, Z0, Z1, S, ctx->ws.as<double>(oGb), ctx->ws.as<double>(oTb), ctx->ws.as<double>(oW2), H,
-              ctx->ws.as<double>(oWp));
-  // descriptor upload: [BlkDesc][GemmDesc][toff][ints]
+  build_dc_gemms(pl, Z0, Z1, S);
+  std::vector<GemmDesc> gdT;
+  build_red_bt_gemms<double>(pl, gdT, H, ctx->ws.as<double>(oWp), S, Z1, Z0, ctx->ws.as<double>(oGb),
+                             ctx->ws.as<double>(oTb), ctx->ws.as<double>(oW2));
+  // descriptor upload: [BlkDesc][GemmDesc D&C][GemmDesc work][toff][ints]
   const size_t bBD = sizeof(BlkDesc) * pl.bd.size(), bP = sizeof(GemmDesc) * pl.gd.size(), bI = sizeof(int) * pl.ibuf.size();
-  const size_t bT = sizeof(int64_t) * pl.toff.size();
-  const size_t oBD = 0, oP = (bBD + 255) & ~(size_t)255, oT = oP + ((bP + 255) & ~(size_t)255);
+  const size_t bQ = sizeof(GemmDesc) * gdT.size(), bT = sizeof(int64_t) * pl.toff.size();
+  const size_t oBD = 0, oP = (bBD + 255) & ~(size_t)255, oQ = oP + ((bP + 255) & ~(size_t)255);
+  const size_t oT = oQ + ((bQ + 255) & ~(size_t)255);
   const size_t oI = oT + ((bT + 255) & ~(size_t)255);
   std::vector<char> blob(oI + bI);
   memcpy(blob.data() + oBD, pl.bd.data(), bBD);
   if (bP) memcpy(blob.data() + oP, pl.gd.data(), bP);
+  if (bQ) memcpy(blob.data() + oQ, gdT.data(), bQ);
   if (bT) memcpy(blob.data() + oT, pl.toff.data(), bT);
   memcpy(blob.data() + oI, pl.ibuf.data(), bI);
   if (int r = ctx->flags.ensure(sizeof(int) * nblk)) return r;
@@ -1689,8 +1979,10 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   SyevLaunch L;
   L.dI = (const int *)(ctx->desc.as<char>(oI));
   L.dG = (const GemmDesc *)(ctx->desc.as<char>(oP));
+  L.dGT = (const void *)(ctx->desc.as<char>(oQ));
   L.toff = (const int64_t *)(ctx->desc.as<char>(oT));
   a.Gb = ctx->ws.as<double>(oGb); a.Tb = ctx->ws.as<double>(oTb);
+  a.Ym = S; a.Yh = Z1;
   L.mst = ctx->ws.as<MergeState>(omst);
   if (int r = launch_core<double>(ctx, pl, a, a, L, st, true, 0)) return r;
   NRG_TIMED(ctx, st, "k_finish", hipLaunchKernelGGL(k_finish<double>, dim3((unsigned)nblk), dim3(256), 0, st, a, L.dI + pl.all_off));
@@ -1704,6 +1996,187 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]);
   ctx->total_ms = ms;
   for (int ph = 0; ph < 4; ph++) {
+    ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[ph], ctx->ev[ph + 1]);
+    ctx->phase_ms[ph] = ms;
+  }
+  return r;
+}
+
+
+// ------------------------------------------------------------------------------------------
+// Mixed form: FP32 reduction and back-transformation on the FP32 matrix cores, FP64 divide and
+// conquer on the tridiagonal matrix, then NREF FP64 refinement sweeps.
+// ------------------------------------------------------------------------------------------
+constexpr int NREF_DEFAULT = 3;   // two sweeps reach FP64 orthogonality but leave ~1e-11 eigenvalue errors
+
+// number of refinement sweeps: NRG_MIXED_SWEEPS in the environment (1..6), default NREF_DEFAULT
+static int refine_sweeps() {
+  const char *e = getenv("NRG_MIXED_SWEEPS");
+  const int v = e ? atoi(e) : NREF_DEFAULT;
+  return v < 1 ? 1 : (v > 6 ? 6 : v);
+}
+
+int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
+                   double *w, int *info, hipStream_t st) {
+  SyevPlan pl;
+  build_plan(pl, nblk, n, off_H, off_w, false, 4);
+  // per-block chunk items for the elementwise refinement kernels
+  const size_t rc_off = pl.ibuf.size();
+  for (int b = 0; b < nblk; b++)
+    for (int64_t c = 0; c < (n[b] * n[b] + COPY_CHUNK - 1) / COPY_CHUNK; c++) { pl.ibuf.push_back(b); pl.ibuf.push_back((int)c); }
+  const size_t rc_cnt = (pl.ibuf.size() - rc_off) / 2;
+  const size_t col_off = pl.ibuf.size();
+  for (int b = 0; b < nblk; b++)
+    for (int64_t c = 0; c < (n[b] + 15) / 16; c++) { pl.ibuf.push_back(b); pl.ibuf.push_back((int)c); }
+  const size_t col_cnt = (pl.ibuf.size() - col_off) / 2;
+  // FP64 workspace
+  Carver cv;
+  const size_t oZ0 = cv.take<double>(pl.sumn2), oZ1 = cv.take<double>(pl.sumn2), oS = cv.take<double>(pl.sumn2);
+  const size_t oAX = cv.take<double>(pl.sumn2), oR = cv.take<double>(pl.sumn2), oE = cv.take<double>(pl.sumn2);
+  const size_t od = cv.take<double>(pl.sumn), oe = cv.take<double>(pl.sumn), ol = cv.take<double>(pl.sumn);
+  const size_t olt = cv.take<double>(pl.sumn);
+  const size_t osc = cv.take<double>(nblk), odscr = cv.take<double>(DS_COUNT * pl.sumn), oiscr = cv.take<int>(IS_COUNT * pl.sumn);
+  const size_t omst = cv.take<MergeState>(pl.mrg_max + 1), oamax = cv.take<double>(nblk), oacc = cv.take<double>(3 * nblk);
+  if (int r = ctx->ws.ensure(cv.off)) return r;
+  // FP32 workspace
+  Carver cf;
+  const size_t fA = cf.take<float>(pl.sumn2), fWp = cf.take<float>(pl.sumWp), fTau = cf.take<float>(pl.sumn);
+  const size_t fTf = cf.take<float>(pl.sumT), fvb = cf.take<float>(pl.sumn), fYp = cf.take<float>(pl.sumYp);
+  const size_t fYm = cf.take<float>(pl.sumn2), fYh = cf.take<float>(pl.sumn2), fZt = cf.take<float>(pl.sumn2);
+  const size_t fGb = cf.take<float>(pl.sumTb + 1), fTb = cf.take<float>(pl.sumTb + 1), fW2 = cf.take<float>(pl.sumW2);
+  if (int r = ctx->ws2.ensure(cf.off)) return r;
+  double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
+  build_dc_gemms(pl, Z0, Z1, S);
+  std::vector<GemmDesc32> gdT;
+  float *A32 = ctx->ws2.as<float>(fA);
+  build_red_bt_gemms<float>(pl, gdT, A32, ctx->ws2.as<float>(fWp), ctx->ws2.as<float>(fYm), ctx->ws2.as<float>(fYh),
+                            ctx->ws2.as<float>(fZt), ctx->ws2.as<float>(fGb), ctx->ws2.as<float>(fTb),
+                            ctx->ws2.as<float>(fW2));
+  // refinement products: AX = A X; S = X^T AX and R = I - X^T X; Xn = X (I + E). Both sweep
+  // directions (X in Z0 -> Xn in Z1, then back) are prepared.
+  std::vector<GemmDesc> gref;
+  double *Xbuf[2] = {Z0, Z1};
+  double *AX = ctx->ws.as<double>(oAX), *Rm = ctx->ws.as<double>(oR), *Em = ctx->ws.as<double>(oE);
+  size_t rg_off[2][4], rg_cnt[2][4];
+  for (int dir = 0; dir < 2; dir++) {
+    double *X = Xbuf[dir], *Xn = Xbuf[1 - dir];
+    for (int ph = 0; ph < 4; ph++) {
+      const size_t d0 = gref.size();
+      for (int b = 0; b < nblk; b++) {
+        const BlkDesc &d = pl.bd[b];
+        const int nb = (int)d.n;
+        GemmDesc g{};
+        g.M = nb; g.N = nb; g.K = nb; g.ldc = nb; g.lda = nb; g.ldb = nb;
+        if (ph == 0) {
+          g.A = H + d.offH; g.B = X + d.offZ; g.C = AX + d.offZ; g.alpha = 1.0; g.beta = 0.0;
+          gref.push_back(g);
+        } else if (ph == 1) {
+          g.A = X + d.offZ; g.tA = 1; g.B = AX + d.offZ; g.C = S + d.offZ; g.alpha = 1.0; g.beta = 0.0;
+          gref.push_back(g);
+          GemmDesc h{};
+          h.M = nb; h.N = nb; h.K = nb; h.ldc = nb; h.lda = nb; h.ldb = nb;
+          h.A = X + d.offZ; h.tA = 1; h.B = X + d.offZ; h.C = Rm + d.offZ; h.alpha = -1.0; h.beta = 1.0;
+          gref.push_back(h);
+        } else if (ph == 2) {
+          g.A = X + d.offZ; g.B = Em + d.offZ; g.C = Xn + d.offZ; g.alpha = 1.0; g.beta = 0.0;
+          gref.push_back(g);
+        } else {   // E += K K / 2
+          g.A = AX + d.offZ; g.B = AX + d.offZ; g.C = Em + d.offZ; g.alpha = 0.5; g.beta = 1.0;
+          gref.push_back(g);
+        }
+      }
+      rg_off[dir][ph] = pl.ibuf.size();
+      gemm_items(gref, d0, gref.size(), pl.ibuf);
+      rg_cnt[dir][ph] = (pl.ibuf.size() - rg_off[dir][ph]) / 2;
+    }
+  }
+  // descriptor blob: [BlkDesc][D&C GemmDesc][FP32 GemmDesc][refinement GemmDesc][toff][ints]
+  const size_t bBD = sizeof(BlkDesc) * pl.bd.size(), bP = sizeof(GemmDesc) * pl.gd.size();
+  const size_t bQ = sizeof(GemmDesc32) * gdT.size(), bR = sizeof(GemmDesc) * gref.size();
+  const size_t bT = sizeof(int64_t) * pl.toff.size(), bI = sizeof(int) * pl.ibuf.size();
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t oBD = 0, oP = al(bBD), oQ = oP + al(bP), oRr = oQ + al(bQ), oT = oRr + al(bR), oI = oT + al(bT);
+  std::vector<char> blob(oI + bI);
+  memcpy(blob.data() + oBD, pl.bd.data(), bBD);
+  if (bP) memcpy(blob.data() + oP, pl.gd.data(), bP);
+  if (bQ) memcpy(blob.data() + oQ, gdT.data(), bQ);
+  if (bR) memcpy(blob.data() + oRr, gref.data(), bR);
+  if (bT) memcpy(blob.data() + oT, pl.toff.data(), bT);
+  memcpy(blob.data() + oI, pl.ibuf.data(), bI);
+  if (int r = ctx->flags.ensure(sizeof(int) * nblk)) return r;
+  if (int r = ctx->hflags.ensure(sizeof(int) * nblk)) return r;
+  NRG_HIP(hipEventRecord(ctx->ev[0], st));
+  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
+  NRG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * nblk, st));
+  NRG_HIP(hipMemsetAsync(ctx->ws.as<double>(oamax), 0, sizeof(double) * nblk, st));
+  const BlkDesc *dbd = (const BlkDesc *)(ctx->desc.as<char>(oBD));
+  const int *I = (const int *)(ctx->desc.as<char>(oI));
+  // FP64 arguments (D&C, scaling), FP32 arguments (reduction, back-transformation)
+  SyevArgs<double> aD{};
+  aD.bd = dbd; aD.A = H; aD.Z0 = Z0; aD.Z1 = Z1; aD.S = S; aD.Zt = Z0;
+  aD.d = ctx->ws.as<double>(od); aD.e = ctx->ws.as<double>(oe); aD.lam = ctx->ws.as<double>(ol);
+  aD.scale = ctx->ws.as<double>(osc); aD.dscr = ctx->ws.as<double>(odscr); aD.iscr = ctx->ws.as<int>(oiscr);
+  aD.Ntot = pl.sumn; aD.flags = (int *)ctx->flags.p; aD.amax = ctx->ws.as<double>(oamax); aD.H = H; aD.w = w;
+  SyevArgs<float> aF{};
+  aF.bd = dbd; aF.A = A32; aF.Wp = ctx->ws2.as<float>(fWp); aF.tau = ctx->ws2.as<float>(fTau);
+  aF.Tf = ctx->ws2.as<float>(fTf); aF.vb = ctx->ws2.as<float>(fvb); aF.Yp = ctx->ws2.as<float>(fYp);
+  aF.Ym = ctx->ws2.as<float>(fYm); aF.Yh = ctx->ws2.as<float>(fYh); aF.Zt = ctx->ws2.as<float>(fZt);
+  aF.Gb = ctx->ws2.as<float>(fGb); aF.Tb = ctx->ws2.as<float>(fTb);
+  aF.Z0 = Z0; aF.Z1 = Z1; aF.S = S; aF.d = aD.d; aF.e = aD.e; aF.lam = aD.lam; aF.scale = aD.scale;
+  aF.dscr = aD.dscr; aF.iscr = aD.iscr; aF.Ntot = pl.sumn; aF.flags = aD.flags; aF.amax = aD.amax; aF.H = H; aF.w = w;
+  SyevLaunch L;
+  L.dI = I;
+  L.dG = (const GemmDesc *)(ctx->desc.as<char>(oP));
+  L.dGT = (const void *)(ctx->desc.as<char>(oQ));
+  L.toff = (const int64_t *)(ctx->desc.as<char>(oT));
+  L.mst = ctx->ws.as<MergeState>(omst);
+  if (int r = launch_core<float>(ctx, pl, aF, aD, L, st, false, 0)) return r;
+  // FP64 copy of the scaled matrix for the residuals (in place in the caller's arena)
+  NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<double>, dim3((unsigned)pl.symm_cnt), dim3(256), 0, st, aD,
+                                                  (const int2 *)(I + pl.symm_off), 1));
+  RefArgs ra;
+  ra.bd = dbd; ra.A = H; ra.AX = AX; ra.S = S; ra.R = Rm; ra.E = Em;
+  ra.lt = ctx->ws.as<double>(olt); ra.acc = ctx->ws.as<double>(oacc); ra.flags = aD.flags;
+  ra.X = Z0; ra.Xn = Z1;
+  const GemmDesc *dR = (const GemmDesc *)(ctx->desc.as<char>(oRr));
+  const int2 *rci = (const int2 *)(I + rc_off), *cci = (const int2 *)(I + col_off);
+  NRG_TIMED(ctx, st, "k_ref_init", hipLaunchKernelGGL(k_ref_init, dim3((unsigned)rc_cnt), dim3(256), 0, st, ra,
+                                                      (const float *)aF.Zt, rci));
+  const int nref = refine_sweeps();
+  for (int sweep = 0; sweep < nref; sweep++) {
+    const int dir = sweep & 1;
+    ra.X = Xbuf[dir];
+    ra.Xn = Xbuf[1 - dir];
+    if (sweep > 0)
+      NRG_TIMED(ctx, st, "k_ref_eye", hipLaunchKernelGGL(k_ref_eye, dim3((unsigned)rc_cnt), dim3(256), 0, st, ra, rci));
+    NRG_HIP(hipMemsetAsync(ra.acc, 0, sizeof(double) * 3 * nblk, st));
+    for (int ph = 0; ph < 2; ph++)
+      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)rg_cnt[dir][ph]), dim3(256), 0, st, dR,
+                                                          (const int2 *)(I + rg_off[dir][ph])));
+    NRG_TIMED(ctx, st, "k_ref_lam", hipLaunchKernelGGL(k_ref_lam, dim3((unsigned)nblk), dim3(256), 0, st, ra, I + pl.all_off));
+    NRG_TIMED(ctx, st, "k_ref_norms", hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
+    NRG_TIMED(ctx, st, "k_ref_E", hipLaunchKernelGGL(k_ref_E, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
+    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)rg_cnt[dir][3]), dim3(256), 0, st, dR,
+                                                        (const int2 *)(I + rg_off[dir][3])));
+    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)rg_cnt[dir][2]), dim3(256), 0, st, dR,
+                                                        (const int2 *)(I + rg_off[dir][2])));
+    NRG_TIMED(ctx, st, "k_ref_clusters", hipLaunchKernelGGL(k_ref_clusters, dim3((unsigned)nblk), dim3(256), 0, st, ra,
+                                                            I + pl.all_off));
+  }
+  NRG_HIP(hipEventRecord(ctx->ev[4], st));
+  ra.X = Xbuf[nref & 1];
+  NRG_TIMED(ctx, st, "k_ref_finish", hipLaunchKernelGGL(k_ref_finish, dim3((unsigned)nblk), dim3(256), 0, st, ra, H, w,
+                                                        (const double *)aD.amax, I + pl.all_off));
+  NRG_TIMED(ctx, st, "k_ref_copy", hipLaunchKernelGGL(k_ref_copy, dim3((unsigned)rc_cnt), dim3(256), 0, st, ra, H, rci));
+  NRG_HIP(hipGetLastError());
+  NRG_HIP(hipEventRecord(ctx->ev[5], st));
+  const int r = finish_flags(ctx, nblk, info, st);
+  if (ctx->kt.on) ctx->kt.collect();
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]);
+  ctx->total_ms = ms;
+  for (int ph = 0; ph < 5; ph++) {
     ms = 0;
     hipEventElapsedTime(&ms, ctx->ev[ph], ctx->ev[ph + 1]);
     ctx->phase_ms[ph] = ms;
