@@ -1,0 +1,228 @@
+// fdm.hip - FDM spectral weights of one NRG step, binned on the logarithmic mesh (nrg_fdm_bin).
+//
+// Replaces, for one step, the pair loop of BaseSpectrum::calc (c++/algo.hpp:55-67) into
+// Algo_FDM::calc = Algo_FDMgt::calc + Algo_FDMls::calc (c++/algo_FDM.hpp:30-60, 84-114, 136-142)
+// and the binning ChainBinning::add -> Bins::add -> Bins::add_std (c++/spectrum.hpp:19-28,
+// c++/bins.hpp:103-132). For a pair (Ii, Ij) with op(j, i) = <j|op|i> (dim_j x dim_i, row-major):
+//
+//   greater:  D_i x D_j, D_i x K_j:  op1 op2 exp(-E_i/T) wnf      K_i x D_j:  op1 (op2 rho_i)
+//   lesser:   D_i x D_j, K_i x D_j:  -sign op1 op2 exp(-E_j/T) wnf   D_i x K_j:  -sign op1 (rho_j op2)
+//
+// each at energy E_j - E_i, times the pair factor. Launch 1 forms the two products (only the
+// sub-blocks the three ranges use) on the FP64 matrix cores; launch 2 evaluates every element of
+// every pair in 64 x 64 tiles (Boltzmann factors of the tile's states in LDS, one log10 per element
+// shared by both halves) and scatters into the bins with FP64 atomics.
+#include "gemm.hpp"
+#include <cmath>
+#include <cstring>
+
+namespace nrg {
+
+struct FdmPairDev {
+  int64_t dim_i, dim_j, K_i, K_j, ld_op;
+  const double *E_i, *E_j, *op1, *op2;
+  const double *op2rho;   // (dim_j - K_j) x K_i, row-major, ld K_i:  (op2 rho_i)(K_j + r, c)
+  const double *rhoop2;   // K_j x (dim_i - K_i), row-major, ld dim_i - K_i:  (rho_j op2)(r, K_i + c)
+  double factor;
+};
+
+struct FdmArgs {
+  const FdmPairDev *pairs;
+  double T, wnf, msign;   // msign = -sign
+  double log10emin, bpd, discard;
+  int64_t nbins;
+  double *gt_pos, *gt_neg, *ls_pos, *ls_neg;
+};
+
+// Bins::add + Bins::add_std (c++/bins.hpp:103-132) with the energy already made non-negative
+__device__ __forceinline__ void bin_add(const FdmArgs &a, double *bins, double energy, double w, int64_t idx,
+                                        double rem, bool low) {
+  if (w == 0.0 || fabs(w) < a.discard * energy) return;
+  if (low) {
+    atomicAdd(bins + idx, w);
+    return;
+  }
+  atomicAdd(bins + idx, (1.0 - rem) * w);
+  atomicAdd(bins + idx + 1, rem * w);
+}
+
+__global__ __launch_bounds__(256) void k_fdm_bin(FdmArgs a, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const FdmPairDev P = a.pairs[it.x];
+  const int ti = (int)((P.dim_i + 63) / 64);
+  const int i0 = (it.y % ti) * 64, j0 = (it.y / ti) * 64;
+  __shared__ double Ei[64], Ej[64], Bi[64], Bj[64];
+  if (threadIdx.x < 64) {
+    const int i = i0 + threadIdx.x;
+    const double e = i < P.dim_i ? P.E_i[i] : 0.0;
+    Ei[threadIdx.x] = e;
+    Bi[threadIdx.x] = exp(-e / a.T);
+  } else if (threadIdx.x < 128) {
+    const int j = j0 + threadIdx.x - 64;
+    const double e = j < P.dim_j ? P.E_j[j] : 0.0;
+    Ej[threadIdx.x - 64] = e;
+    Bj[threadIdx.x - 64] = exp(-e / a.T);
+  }
+  __syncthreads();
+  const int il = threadIdx.x & 63;
+  const int i = i0 + il;
+  if (i >= P.dim_i) return;
+  const bool Di = i >= P.K_i;
+  for (int jl = threadIdx.x >> 6; jl < 64; jl += 4) {
+    const int j = j0 + jl;
+    if (j >= P.dim_j) break;
+    const bool Dj = j >= P.K_j;
+    if (!Di && !Dj) continue;
+    const double o1 = P.op1[(size_t)j * P.ld_op + i];
+    const double o2 = P.op2[(size_t)j * P.ld_op + i];
+    double wgt, wls;
+    if (Di) wgt = o1 * o2 * Bi[il] * a.wnf;   // evaluation order of the reference terms
+    else wgt = o1 * P.op2rho[(size_t)(j - P.K_j) * P.K_i + i];
+    if (Dj) wls = o1 * o2 * a.msign * Bj[jl] * a.wnf;
+    else wls = o1 * P.rhoop2[(size_t)j * (P.dim_i - P.K_i) + (i - P.K_i)] * a.msign;
+    wgt *= P.factor;
+    wls *= P.factor;
+    const double de = Ej[jl] - Ei[il];
+    const bool pos = de >= 0.0;
+    const double energy = pos ? de : -de;
+    // bin position, shared by both halves (zero_epsilon = 1e-14, c++/bins.hpp:46)
+    int64_t idx = 0;
+    double rem = 0.0;
+    bool low = true;
+    if (energy >= 1e-14) {
+      const double x = (log10(energy) - a.log10emin) * a.bpd;
+      const double ip = floor(x);
+      if (ip < 0) {
+        idx = 0;
+      } else if (ip >= (double)(a.nbins - 1)) {
+        idx = a.nbins - 1;
+      } else {
+        idx = (int64_t)ip;
+        rem = x - ip;
+        low = false;
+      }
+    }
+    bin_add(a, pos ? a.gt_pos : a.gt_neg, energy, wgt, idx, rem, low);
+    bin_add(a, pos ? a.ls_pos : a.ls_neg, energy, wls, idx, rem, low);
+  }
+}
+
+int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf,
+            int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, hipStream_t st) {
+  if (npairs < 0 || (npairs > 0 && !pairs) || !grid || grid->nbins < 2 || !(T > 0) || (sign != 1 && sign != -1) ||
+      !gt_pos || !gt_neg || !ls_pos || !ls_neg || !(grid->bins_per_decade > 0)) {
+    set_error("nrg_fdm_bin: invalid arguments");
+    return NRG_E_ARG;
+  }
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair &q = pairs[p];
+    if (q.dim_i < 0 || q.dim_j < 0 || q.K_i < 0 || q.K_j < 0 || q.K_i > q.dim_i || q.K_j > q.dim_j ||
+        (q.dim_i && q.dim_j && (!q.op1 || !q.op2 || !q.E_i || !q.E_j || q.ld_op < q.dim_i))) {
+      set_error("nrg_fdm_bin: pair " + std::to_string(p) + " is inconsistent");
+      return NRG_E_ARG;
+    }
+    if (q.dim_i > INT32_MAX / 2 || q.dim_j > INT32_MAX / 2 || q.ld_op > INT32_MAX || q.n_rho_i > INT32_MAX ||
+        q.n_rho_j > INT32_MAX) {
+      set_error("nrg_fdm_bin: dimension too large");
+      return NRG_E_ARG;
+    }
+    // the density-matrix products need rho on every kept state they touch
+    const bool need_i = q.K_i > 0 && q.dim_j > q.K_j, need_j = q.K_j > 0 && q.dim_i > q.K_i;
+    if ((need_i && (!q.rho_i || q.n_rho_i < q.K_i)) || (need_j && (!q.rho_j || q.n_rho_j < q.K_j))) {
+      set_error("nrg_fdm_bin: pair " + std::to_string(p) + " needs a density matrix covering its kept states");
+      return NRG_E_ARG;
+    }
+  }
+  // product buffers
+  std::vector<int64_t> o_or(npairs, -1), o_ro(npairs, -1);
+  int64_t tot = 0;
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair &q = pairs[p];
+    if (q.K_i > 0 && q.dim_j > q.K_j) { o_or[p] = tot; tot += (q.dim_j - q.K_j) * q.K_i; }
+    if (q.K_j > 0 && q.dim_i > q.K_i) { o_ro[p] = tot; tot += q.K_j * (q.dim_i - q.K_i); }
+  }
+  if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
+  double *prod = ctx->ws2.as<double>();
+  std::vector<GemmDesc> gd;
+  std::vector<FdmPairDev> pd(npairs);
+  std::vector<int> items;
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair &q = pairs[p];
+    FdmPairDev &d = pd[p];
+    d.dim_i = q.dim_i; d.dim_j = q.dim_j; d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
+    d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
+    d.op2rho = o_or[p] >= 0 ? prod + o_or[p] : nullptr;
+    d.rhoop2 = o_ro[p] >= 0 ? prod + o_ro[p] : nullptr;
+    if (o_or[p] >= 0) {
+      // (op2 rho_i)[K_j:, :K_i] = op2[K_j:, :c] rho_i[:c, :K_i], c = min(n_rho_i, dim_i) (prod_fit)
+      const int64_t c = std::min(q.n_rho_i, q.dim_i);
+      // row-major C (R x K_i) = X (R x c) * Y (c x K_i): column-major C^T = Y^T X^T
+      GemmDesc g{};
+      g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 0;
+      g.B = q.op2 + (size_t)q.K_j * q.ld_op; g.ldb = (int)q.ld_op; g.tB = 0;
+      g.C = (double *)d.op2rho; g.ldc = (int)q.K_i;
+      g.M = (int)q.K_i; g.N = (int)(q.dim_j - q.K_j); g.K = (int)c; g.alpha = 1.0; g.beta = 0.0;
+      gd.push_back(g);
+    }
+    if (o_ro[p] >= 0) {
+      // (rho_j op2)[:K_j, K_i:] = rho_j[:K_j, :c] op2[:c, K_i:], c = min(n_rho_j, dim_j)
+      const int64_t c = std::min(q.n_rho_j, q.dim_j);
+      GemmDesc g{};
+      g.A = q.op2 + q.K_i; g.lda = (int)q.ld_op; g.tA = 0;
+      g.B = q.rho_j; g.ldb = (int)q.n_rho_j; g.tB = 0;
+      g.C = (double *)d.rhoop2; g.ldc = (int)(q.dim_i - q.K_i);
+      g.M = (int)(q.dim_i - q.K_i); g.N = (int)q.K_j; g.K = (int)c; g.alpha = 1.0; g.beta = 0.0;
+      gd.push_back(g);
+    }
+    // element tiles, skipping those that lie entirely in K_i x K_j
+    const int ti = (int)((q.dim_i + 63) / 64), tj = (int)((q.dim_j + 63) / 64);
+    for (int b = 0; b < tj; b++)
+      for (int a = 0; a < ti; a++) {
+        const bool allK = (int64_t)(a + 1) * 64 <= q.K_i && (int64_t)(b + 1) * 64 <= q.K_j;
+        if (allK) continue;
+        items.push_back(p);
+        items.push_back(b * ti + a);
+      }
+  }
+  std::vector<int> gitems;
+  gemm_items(gd, 0, gd.size(), gitems);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t bG = sizeof(GemmDesc) * gd.size(), bPd = sizeof(FdmPairDev) * pd.size();
+  const size_t oG = 0, oPd = al(bG), oGI = oPd + al(bPd), oI = oGI + al(sizeof(int) * gitems.size());
+  const size_t total = oI + sizeof(int) * items.size() + 8;
+  std::vector<char> blob(total);
+  if (bG) memcpy(blob.data() + oG, gd.data(), bG);
+  if (bPd) memcpy(blob.data() + oPd, pd.data(), bPd);
+  if (!gitems.empty()) memcpy(blob.data() + oGI, gitems.data(), sizeof(int) * gitems.size());
+  if (!items.empty()) memcpy(blob.data() + oI, items.data(), sizeof(int) * items.size());
+  NRG_HIP(hipEventRecord(ctx->ev[0], st));
+  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
+  if (!gitems.empty())
+    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(gitems.size() / 2)), dim3(256), 0, st,
+                                                        (const GemmDesc *)ctx->desc.as<char>(oG),
+                                                        (const int2 *)ctx->desc.as<char>(oGI)));
+  NRG_HIP(hipEventRecord(ctx->ev[1], st));
+  FdmArgs fa;
+  fa.pairs = (const FdmPairDev *)ctx->desc.as<char>(oPd);
+  fa.T = T; fa.wnf = wnf; fa.msign = -(double)sign;
+  fa.log10emin = grid->log10emin; fa.bpd = grid->bins_per_decade; fa.discard = grid->discard_immediately;
+  fa.nbins = grid->nbins;
+  fa.gt_pos = gt_pos; fa.gt_neg = gt_neg; fa.ls_pos = ls_pos; fa.ls_neg = ls_neg;
+  if (!items.empty())
+    NRG_TIMED(ctx, st, "k_fdm_bin", hipLaunchKernelGGL(k_fdm_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st, fa,
+                                                       (const int2 *)ctx->desc.as<char>(oI)));
+  NRG_HIP(hipGetLastError());
+  NRG_HIP(hipEventRecord(ctx->ev[2], st));
+  NRG_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]);
+  ctx->total_ms = ms;
+  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+  ctx->phase_ms[0] = ms;
+  hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
+  ctx->phase_ms[1] = ms;
+  if (ctx->kt.on) ctx->kt.collect();
+  return NRG_OK;
+}
+
+}  // namespace nrg

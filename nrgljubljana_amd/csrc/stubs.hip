@@ -5,9 +5,4 @@ int gemm3_ozaki(nrg_ctx *, const nrg_gemm3_term *, int, hipStream_t) {
   set_error("nrg_gemm3_grouped: split-precision form not built yet");
   return NRG_E_ARG;
 }
-int fdm_run(nrg_ctx *, const nrg_fdm_pair *, int, const nrg_bin_grid *, double, double, int, double *, double *,
-            double *, double *, hipStream_t) {
-  set_error("nrg_fdm_bin: not built yet");
-  return NRG_E_ARG;
-}
 }  // namespace nrg
