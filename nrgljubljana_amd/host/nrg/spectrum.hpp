@@ -1,0 +1,133 @@
+// spectrum.hpp - FDM spectral accumulation of one NRG step on the GPU.
+//
+// BinGrid follows Bins::setlimits / loggrid_std (c++/bins.hpp:61-100); FDMSpectrum is BaseSpectrum::calc
+// (c++/algo.hpp:55-67) with Algo_FDM (c++/algo_FDM.hpp:123-151) and its two ChainBinning accumulators
+// (greater and lesser halves, c++/spectrum.hpp:12-31): every subspace pair of a step with both operator
+// blocks present is handed to one nrg_fdm_bin call; the step's bins are merged into the running
+// spectrum (SpectrumRealFreq::mergeCFS, c++/spectrum.hpp:73-76) on the host.
+#pragma once
+#include "dmnrg.hpp"
+
+namespace NRG {
+
+struct BinGrid {
+  double log10emin = 0, log10emax = 0;
+  double bins = 1000;   // bins per decade (P.bins)
+  size_t nbins = 0;
+  double discard_immediately = 1e-16;
+  // emin / emax as Bins::setlimits computes them from P.emin, P.emax, SCALE(0), last_step_scale()
+  static BinGrid standard(double emin_param, double emax_param, double scale0, double last_step_scale, double bins,
+                          double discard_immediately = 1e-16) {
+    const double emax = emax_param > 0 ? emax_param : scale0 * std::pow(10.0, 2.0);
+    const double emin = emin_param > 0 ? emin_param : last_step_scale / std::pow(10.0, 2.0);
+    BinGrid g;
+    g.log10emin = std::floor(std::log10(emin));
+    g.log10emax = std::ceil(std::log10(emax));
+    g.bins = bins;
+    g.nbins = (size_t)((g.log10emax - g.log10emin) * bins + 1.0);
+    g.discard_immediately = discard_immediately;
+    return g;
+  }
+  double energy(size_t i) const { return std::pow(10.0, log10emin + (double)i / bins); }
+  nrg_bin_grid c() const { return nrg_bin_grid{log10emin, bins, (int64_t)nbins, discard_immediately}; }
+};
+
+// Spectrum of one algorithm half: weights on the grid for positive and negative energies.
+struct SpectrumBins {
+  std::vector<double> pos, neg;
+  explicit SpectrumBins(size_t n = 0) : pos(n, 0.0), neg(n, 0.0) {}
+  double total_weight() const {
+    double s = 0;
+    for (double x : pos) s += x;
+    for (double x : neg) s += x;
+    return s;
+  }
+};
+
+using FactorFnc = std::function<double(const Invar &, const Invar &)>;   // ff(Ii, Ij)
+using CheckFnc = std::function<bool(const Invar &, const Invar &, int)>;  // cf(Ij, Ii, spin)
+
+class FDMSpectrum {
+ public:
+  FDMSpectrum(Device &dev, const BinGrid &grid, int sign, double T)
+      : dev_(dev), grid_(grid), sign_(sign), T_(T), acc_(4 * grid.nbins), gt_(grid.nbins), ls_(grid.nbins) {
+    NRG_ASSERT(sign == 1 || sign == -1);
+  }
+  // One step: energies abs_G of `diag`, operator blocks op(Ij, Ii) = <Ij|op|Ii> (keys {Ij, Ii}),
+  // rhoFDM of the same step, wnf = stats.wnfactor[N].
+  void calc(const DiagInfo &diag, const MatrixElements &op1, const MatrixElements &op2, const DensMatElements &rhoFDM,
+            double wnf, const FactorFnc &ff, const CheckFnc &cf = nullptr, int spin = 0) {
+    acc_.zero();
+    // energies of every subspace, in device memory
+    std::map<Invar, std::pair<size_t, size_t>> eoff;
+    std::vector<double> E;
+    for (const auto &[I, eig] : diag) {
+      const auto e = eig.values.all_abs_G();
+      eoff[I] = {E.size(), e.size()};
+      E.insert(E.end(), e.begin(), e.end());
+    }
+    DBuf<double> dE(E.size() + 1);
+    if (!E.empty())
+      hip_check(hipMemcpy(dE.get(), E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    std::vector<nrg_fdm_pair> pairs;
+    for (const auto &[Ii, diagi] : diag)
+      for (const auto &[Ij, diagj] : diag) {
+        const Twoinvar II{Ij, Ii};
+        const auto a = op1.find(II), b = op2.find(II);
+        if (a == op1.end() || b == op2.end() || (cf && !cf(Ij, Ii, spin))) continue;
+        const DMatrix &o1 = a->second, &o2 = b->second;
+        NRG_ASSERT(size1(o1) == size1(o2) && size2(o1) == size2(o2));
+        nrg_fdm_pair p{};
+        p.dim_i = (int64_t)size2(o1);
+        p.dim_j = (int64_t)size1(o1);
+        NRG_ASSERT((size_t)p.dim_i <= diagi.getnrall() && (size_t)p.dim_j <= diagj.getnrall());
+        p.K_i = (int64_t)std::min(diagi.Krange().second, (size_t)p.dim_i);
+        p.K_j = (int64_t)std::min(diagj.Krange().second, (size_t)p.dim_j);
+        p.E_i = dE.get() + eoff[Ii].first;
+        p.E_j = dE.get() + eoff[Ij].first;
+        p.op1 = o1.data();
+        p.op2 = o2.data();
+        p.ld_op = (int64_t)size2(o1);
+        if (const auto r = rhoFDM.find(Ii); r != rhoFDM.end()) { p.rho_i = r->second.data(); p.n_rho_i = (int64_t)size1(r->second); }
+        if (const auto r = rhoFDM.find(Ij); r != rhoFDM.end()) { p.rho_j = r->second.data(); p.n_rho_j = (int64_t)size1(r->second); }
+        p.factor = ff(Ii, Ij);
+        pairs.push_back(p);
+      }
+    const nrg_bin_grid g = grid_.c();
+    double *base = acc_.get();
+    const size_t nb = grid_.nbins;
+    nrg_check(nrg_fdm_bin(dev_.ctx(), pairs.data(), (int)pairs.size(), &g, T_, wnf, sign_, base, base + nb, base + 2 * nb,
+                          base + 3 * nb, dev_.stream()),
+              "nrg_fdm_bin");
+    // mergeCFS of the step's chains into the running spectra
+    std::vector<double> h(4 * nb);
+    hip_check(hipMemcpy(h.data(), base, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    for (size_t i = 0; i < nb; i++) {
+      gt_.pos[i] += h[i];
+      gt_.neg[i] += h[nb + i];
+      ls_.pos[i] += h[2 * nb + i];
+      ls_.neg[i] += h[3 * nb + i];
+    }
+  }
+  const SpectrumBins &gt() const { return gt_; }   // FDMgt
+  const SpectrumBins &ls() const { return ls_; }   // FDMls
+  SpectrumBins total() const {                     // FDM = gt + ls
+    SpectrumBins t(grid_.nbins);
+    for (size_t i = 0; i < grid_.nbins; i++) {
+      t.pos[i] = gt_.pos[i] + ls_.pos[i];
+      t.neg[i] = gt_.neg[i] + ls_.neg[i];
+    }
+    return t;
+  }
+  const BinGrid &grid() const { return grid_; }
+
+ private:
+  Device &dev_;
+  BinGrid grid_;
+  int sign_;
+  double T_;
+  DBuf<double> acc_;
+  SpectrumBins gt_, ls_;
+};
+
+}  // namespace NRG
