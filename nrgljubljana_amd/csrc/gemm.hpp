@@ -50,6 +50,17 @@ inline void gemm_items(const std::vector<GemmDescT<T>> &d, size_t d0, size_t d1,
   }
 }
 
+// The two stages of a grouped triple-product table (nrg_gemm3_grouped): gd[0, nd1) form the
+// intermediates, gd[nd1, ...) accumulate into each distinct C with K-parts
+// parts[prange[g - nd1].first, .second) (the descriptors' parts pointers are set by the executor).
+struct Gemm3Plan {
+  std::vector<GemmDesc> gd;
+  size_t nd1 = 0;
+  std::vector<GemmPart> parts;
+  std::vector<std::pair<size_t, size_t>> prange;
+};
+int gemm3_plan(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, Gemm3Plan &pl);
+
 __global__ void k_gemm_f64(const GemmDesc *descs, const int2 *items);
 __global__ void k_gemm_f32(const GemmDesc32 *descs, const int2 *items);
 

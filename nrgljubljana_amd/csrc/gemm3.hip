@@ -47,8 +47,9 @@ GemmPart rm_part(RmOperand X, RmOperand Y, int64_t K, double scale) {
 
 }  // namespace
 
-int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_t st) {
-  // intermediates
+// Builds the two stages of a table: gd[0, nd1) form the intermediates (stored in ws2), gd[nd1, ...)
+// accumulate into each distinct C, whose K-parts are parts[prange[g].first, prange[g].second).
+int gemm3_plan(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, Gemm3Plan &pl) {
   std::vector<int64_t> toff(nterms, -1);
   int64_t tsum = 0;
   for (int i = 0; i < nterms; i++) {
@@ -59,18 +60,19 @@ int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_
   }
   if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tsum + 1))) return r;
   double *Tbase = ctx->ws2.as<double>();
-  std::vector<GemmDesc> gd;
-  std::vector<GemmPart> parts;
+  pl.gd.clear();
+  pl.parts.clear();
+  pl.prange.clear();
   for (int i = 0; i < nterms; i++) {
     if (toff[i] < 0) continue;
     const nrg_gemm3_term &t = terms[i];
     double *T = Tbase + toff[i];
     if (t.transA)   // T = A^T O  (m x l)
-      gd.push_back(rm_desc(T, t.l, {t.A, t.lda, 1}, {t.O, t.ldo, 0}, t.m, t.l, t.k, 1.0, 0.0));
+      pl.gd.push_back(rm_desc(T, t.l, {t.A, t.lda, 1}, {t.O, t.ldo, 0}, t.m, t.l, t.k, 1.0, 0.0));
     else            // T = O opB  (k x n)
-      gd.push_back(rm_desc(T, t.n, {t.O, t.ldo, 0}, {t.B, t.ldb, !t.transB}, t.k, t.n, t.l, 1.0, 0.0));
+      pl.gd.push_back(rm_desc(T, t.n, {t.O, t.ldo, 0}, {t.B, t.ldb, !t.transB}, t.k, t.n, t.l, 1.0, 0.0));
   }
-  const size_t nd1 = gd.size();
+  pl.nd1 = pl.gd.size();
   // grouped accumulation, one descriptor per distinct C
   std::map<double *, std::vector<int>> groups;
   std::vector<double *> order;
@@ -80,10 +82,9 @@ int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_
     if (!groups.count(t.C)) order.push_back(t.C);
     groups[t.C].push_back(i);
   }
-  std::vector<std::pair<size_t, size_t>> prange;   // parts of each group descriptor
   for (double *C : order) {
     const std::vector<int> &g = groups[C];
-    const size_t p0 = parts.size();
+    const size_t p0 = pl.parts.size();
     for (int i : g) {
       const nrg_gemm3_term &t = terms[i];
       if (t.m != terms[g[0]].m || t.n != terms[g[0]].n || t.ldc != terms[g[0]].ldc) {
@@ -91,17 +92,26 @@ int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_
         return NRG_E_ARG;
       }
       const RmOperand opB{t.B, t.ldb, !t.transB};
-      if (!t.O) parts.push_back(rm_part({t.A, t.lda, t.transA}, opB, t.k, t.f));
-      else if (t.transA) parts.push_back(rm_part({Tbase + toff[i], t.l, 0}, opB, t.l, t.f));
-      else parts.push_back(rm_part({t.A, t.lda, 0}, {Tbase + toff[i], t.n, 0}, t.k, t.f));
+      if (!t.O) pl.parts.push_back(rm_part({t.A, t.lda, t.transA}, opB, t.k, t.f));
+      else if (t.transA) pl.parts.push_back(rm_part({Tbase + toff[i], t.l, 0}, opB, t.l, t.f));
+      else pl.parts.push_back(rm_part({t.A, t.lda, 0}, {Tbase + toff[i], t.n, 0}, t.k, t.f));
     }
     const nrg_gemm3_term &t0 = terms[g[0]];
     GemmDesc d{};
     d.C = C; d.ldc = (int)t0.ldc; d.M = (int)t0.n; d.N = (int)t0.m;
-    d.alpha = 1.0; d.beta = 1.0; d.nparts = (int)(parts.size() - p0);
-    gd.push_back(d);
-    prange.push_back({p0, parts.size()});
+    d.alpha = 1.0; d.beta = 1.0; d.nparts = (int)(pl.parts.size() - p0);
+    pl.gd.push_back(d);
+    pl.prange.push_back({p0, pl.parts.size()});
   }
+  return NRG_OK;
+}
+
+int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_t st) {
+  Gemm3Plan pl;
+  if (int r = gemm3_plan(ctx, terms, nterms, pl)) return r;
+  std::vector<GemmDesc> &gd = pl.gd;
+  const size_t nd1 = pl.nd1;
+  std::vector<GemmPart> &parts = pl.parts;
   if (gd.empty()) {
     ctx->total_ms = 0;
     return NRG_OK;
@@ -116,7 +126,7 @@ int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_
   const size_t total = oI2 + sizeof(int) * items2.size();
   if (int r = ctx->desc.ensure(total)) return r;
   const GemmPart *dparts = (const GemmPart *)(ctx->desc.as<char>(oP));
-  for (size_t gi = nd1; gi < gd.size(); gi++) gd[gi].parts = dparts + prange[gi - nd1].first;
+  for (size_t gi = nd1; gi < gd.size(); gi++) gd[gi].parts = dparts + pl.prange[gi - nd1].first;
   std::vector<char> blob(total);
   memcpy(blob.data() + oD, gd.data(), bD);
   if (bP) memcpy(blob.data() + oP, parts.data(), bP);

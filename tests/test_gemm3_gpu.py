@@ -95,3 +95,73 @@ def test_bad_terms_are_rejected(ctx):
     t.l = 5   # O NULL requires k == l
     with pytest.raises(NrgError):
         ctx.gemm3_grouped([t])
+
+
+# ---- split-precision form (NRG_GEMM_OZAKI): BF16 slices on the matrix cores, FP32 exact partial
+# sums, FP64 accumulation. Bar: normwise ||C - C_exact||_F <= 1e-12 * sum_terms |f| ||A|| ||O|| ||B||
+# (Frobenius norms), the C4 criterion ||C - C_64||_F / (||U_I|| ||O|| ||U_J||) <= 1e-12.
+
+def norm_scale(terms, C):
+    s = 0.0
+    for t in terms:
+        if t["C"] is C:
+            s += abs(t["f"]) * np.linalg.norm(t["A"].view()) * np.linalg.norm(t["B"].view()) * \
+                (np.linalg.norm(t["O"].view()) if t["O"] is not None else 1.0)
+    return s
+
+
+@pytest.mark.parametrize("seed", [31, 32, 33])
+def test_ozaki_random_tables(ctx, seed):
+    from nrgljubljana_amd import NRG_GEMM_OZAKI
+    rng = np.random.default_rng(seed)
+    outs, terms = random_table(rng, nout=8, maxdim=150)
+    print(f"ozaki random table seed {seed}: {len(terms)} terms", flush=True)
+    gpu, ref, ref_ld = run_both(ctx, outs, terms, NRG_GEMM_OZAKI)
+    for C, g, e in zip(outs, gpu, ref_ld):
+        s = norm_scale(terms, C) + 1e-300
+        err = np.linalg.norm(g - e) / s
+        assert err <= 1e-12, err
+
+
+def test_ozaki_recalc_shapes_and_scales(ctx):
+    # U blocks with orthonormal rows (columns of eigenvector matrices), O with entries spread over
+    # many decades, several table rows per output
+    from nrgljubljana_amd import NRG_GEMM_OZAKI
+    rng = np.random.default_rng(41)
+    outs, terms = [], []
+    for dim1, dimp, r1, rp in [(700, 650, 300, 280), (1000, 900, 500, 420), (64, 1500, 30, 700)]:
+        C = Mat(rng, dim1, dimp, fill=False)
+        outs.append(C)
+        for _ in range(3):
+            U1 = Mat(rng, dim1, r1, pad=17)
+            q, _ = np.linalg.qr(rng.standard_normal((dim1, r1)))
+            U1.view()[:] = q
+            Up = Mat(rng, dimp, rp, pad=5)
+            q, _ = np.linalg.qr(rng.standard_normal((dimp, rp)))
+            Up.view()[:] = q
+            O = Mat(rng, r1, rp)
+            O.view()[:] *= 10.0 ** rng.uniform(-6, 3, size=(r1, rp))
+            terms.append(term("transform", C, U1, Up, O, f=float(rng.uniform(-1, 1))))
+    print("ozaki recalc shapes", flush=True)
+    gpu, ref, ref_ld = run_both(ctx, outs, terms, NRG_GEMM_OZAKI)
+    for C, g, e in zip(outs, gpu, ref_ld):
+        err = np.linalg.norm(g - e) / norm_scale(terms, C)
+        assert err <= 1e-12, err
+
+
+def test_ozaki_rotate_identity_and_zero_blocks(ctx):
+    from nrgljubljana_amd import NRG_GEMM_OZAKI
+    rng = np.random.default_rng(42)
+    n = 257
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    U = Mat(rng, n, n)
+    U.view()[:] = q
+    O = Mat(rng, n, n)
+    O.view()[:] = np.eye(n)
+    Z = Mat(rng, n, n)
+    Z.view()[:] = 0.0   # an all-zero operand splits into zero slices
+    C = Mat(rng, n, n, fill=False)
+    C2 = Mat(rng, n, n, fill=False)
+    gpu, _, _ = run_both(ctx, [C, C2], [term("rotate", C, U, U, O), term("rotate", C2, U, U, Z)], NRG_GEMM_OZAKI)
+    assert np.max(np.abs(gpu[0] - np.eye(n))) < 1e-13
+    assert np.max(np.abs(gpu[1])) == 0.0
