@@ -105,24 +105,12 @@ def cpu_baseline(sizes, off_H, arena, off_w, flops, budget_s=30.0):
                       + ", ".join(f"{k}={v:.3f}" for k, v in res.items())}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--nblk", type=int, default=256)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-check", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the eigensolver shards by blocks across GPUs in "
-              "design only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def bench_eig(args):
     import torch
     from nrgljubljana_amd import Context, NRG_SYEV_FP64, NRG_SYEV_MIXED, workloads
 
+    if args.mode not in ("fp64", "mixed"):
+        sys.exit("bench.py: the eigensolver forms are fp64 and mixed")
     mode = NRG_SYEV_FP64 if args.mode == "fp64" else NRG_SYEV_MIXED
     t0 = time.perf_counter()
     sizes, off_H, arena, off_w, lams = workloads.eig_batch(nblk=args.nblk)
@@ -233,6 +221,232 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sizes, off_H, arena, off_w, flops)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def timed_calls(step, steps, warmup, sync):
+    for _ in range(warmup):
+        step()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sync()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def bench_recalc(args):
+    """Grouped U^T O U recalculation (recalc_general over 16-row two-channel QSZ doublet tables):
+    FP64 and split-precision forms on the same table; value = FP64-equivalent GFLOP/s of --mode."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, NRG_GEMM_FP64, NRG_GEMM_OZAKI, Gemm3Term, workloads
+    t0 = time.perf_counter()
+    table = workloads.recalc_table(npairs=args.npairs)
+    f1, f2 = workloads.recalc_flops(table)
+    flops = f1 + f2
+    log(f"recalc table: {len(table)} outputs x 16 rows, {flops / 1e9:.2f} GFLOP per call, "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    keep, terms, outs = [], [], []
+    for t in table:
+        U1 = torch.from_numpy(t["U1"]).to(dev)
+        Up = torch.from_numpy(t["Up"]).to(dev)
+        C = torch.zeros(t["dim1"], t["dimp"], dtype=torch.float64, device=dev)
+        keep += [U1, Up]
+        outs.append(C)
+        o1 = op = 0
+        for i, (a, b) in enumerate(zip(t["r1"], t["rp"])):
+            if a and b:
+                O = torch.from_numpy(t["O"][i]).to(dev)
+                keep.append(O)
+                s = Gemm3Term()
+                s.m, s.n, s.k, s.l, s.f = t["dim1"], t["dimp"], int(a), int(b), float(t["f"][i])
+                s.A, s.lda, s.transA = U1.data_ptr() + 8 * o1, t["dim1"], 0
+                s.O, s.ldo = O.data_ptr(), int(b)
+                s.B, s.ldb, s.transB = Up.data_ptr() + 8 * op, t["dimp"], 0
+                s.C, s.ldc = C.data_ptr(), t["dimp"]
+                terms.append(s)
+            o1 += int(a)
+            op += int(b)
+    ctx = Context(0)
+    res, kts = {}, {}
+    for name, prec in (("fp64", NRG_GEMM_FP64), ("ozaki", NRG_GEMM_OZAKI)):
+        def step():
+            for C in outs:
+                C.zero_()
+            ctx.gemm3_grouped(terms, prec)
+        ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+        ctx.kernel_timing(True)
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        kts[name] = ctx.kernel_times()
+        ctx.kernel_timing(False)
+        res[name] = (ms, [C.cpu().numpy() for C in outs])
+        log(f"  {name}: {ms:.3f} ms per call, {flops / ms / 1e6:.1f} GFLOP/s (FP64-equivalent); kernels "
+            + ", ".join(f"{k} {v[0] / 2:.3f} ms" for k, v in kts[name].items()))
+    # normwise error of the split-precision form against FP64, per output, scaled as the C4 criterion
+    worst = 0.0
+    for t, cf, co in zip(table, res["fp64"][1], res["ozaki"][1]):
+        o1 = op = 0
+        scale = 0.0
+        for i, (a, b) in enumerate(zip(t["r1"], t["rp"])):
+            if a and b:
+                scale += np.linalg.norm(t["U1"][:, o1:o1 + a]) * np.linalg.norm(t["O"][i]) * \
+                    np.linalg.norm(t["Up"][:, op:op + b])
+            o1 += int(a)
+            op += int(b)
+        worst = max(worst, float(np.linalg.norm(co - cf)) / scale)
+    mode = args.mode if args.mode in ("fp64", "ozaki") else "ozaki"
+    ms = res[mode][0]
+    kname = "k_gemm_f64" if mode == "fp64" else "k_oz_gemm"
+    tms, cnt = kts[mode][kname]
+    ach = flops / (tms / cnt * 1e-3 * 2) / 1e12 if cnt else None   # two launches per call
+    roof = {"bound": "mfma", "kernel": kname, "achieved": round(ach, 3), "unit": "TFLOP/s", "traffic": None}
+    if mode == "fp64":
+        roof.update(peak=FP64_PEAK_TFLOPS, frac=round(ach / FP64_PEAK_TFLOPS, 4))
+    else:
+        slice_tf = ach * 28   # 28 BF16 slice products per FP64 product
+        roof.update(peak=BF16_PEAK_TFLOPS, frac=round(slice_tf / BF16_PEAK_TFLOPS, 4),
+                    note="achieved is FP64-equivalent; frac is the BF16 slice-product rate (28 slice "
+                         "products per FP64 product) over the 2.5 PFLOP/s dense BF16 peak")
+    cpu = None
+    if not args.no_cpu:
+        # oracle dgemm on the host cores over a bounded sample of the table
+        L = oracle.lib()
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        L.oracle_blas_threads(cores)
+        sample = table
+        sf = flops
+        hterms, hold = [], []
+        for t in sample:
+            C = np.zeros((t["dim1"], t["dimp"]))
+            U1, Up = np.ascontiguousarray(t["U1"]), np.ascontiguousarray(t["Up"])
+            hold += [C, U1, Up]
+            o1 = op = 0
+            for i, (a, b) in enumerate(zip(t["r1"], t["rp"])):
+                if a and b:
+                    s = Gemm3Term()
+                    s.m, s.n, s.k, s.l, s.f = t["dim1"], t["dimp"], int(a), int(b), float(t["f"][i])
+                    s.A, s.lda, s.transA = U1.ctypes.data + 8 * o1, t["dim1"], 0
+                    O = np.ascontiguousarray(t["O"][i])
+                    hold.append(O)
+                    s.O, s.ldo = O.ctypes.data, int(b)
+                    s.B, s.ldb, s.transB = Up.ctypes.data + 8 * op, t["dimp"], 0
+                    s.C, s.ldc = C.ctypes.data, t["dimp"]
+                    hterms.append(s)
+                o1 += int(a)
+                op += int(b)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 200:
+            oracle.gemm3(hterms)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * sf / dt / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+               "sample": f"the whole table ({len(table)} outputs x 16 rows) {reps} times, oracle dgemm pairs in the "
+                         f"reference's order, {dt:.1f} s"}
+    out = {"metric": "batched U^T O U recalc GFLOP/s (FP64-equivalent), split-precision vs FP64 at 1e-12 normwise",
+           "value": round(flops / ms / 1e6, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16-slices+f32+f64" if mode == "ozaki" else "f64", "data": "synthetic",
+           "config": {"workload": f"recalc_general, {len(table)} outputs, dims log-uniform [16,2000], 16-row QSZ "
+                                  "2ch doublet tables, Haar U, N(0,1) O, seed 20261205", "form": mode,
+                      "flops_per_step": flops},
+           "fp64_ms": round(res["fp64"][0], 3), "ozaki_ms": round(res["ozaki"][0], 3),
+           "ozaki_vs_fp64_normwise_err": worst, "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_fdm(args):
+    """FDM spectral weights + log binning of one synthetic step (nrg_fdm_bin); value = elements/s."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, FdmPair, BinGrid, workloads
+    pairs = workloads.fdm_step(npairs=args.npairs)
+    nel = workloads.fdm_elements(pairs)
+    dev = torch.device("cuda:0")
+    grid = BinGrid(-12.0, 1000.0, 14001, 1e-16)
+    keep, structs = [], []
+    for p in pairs:
+        t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+        keep.append(t)
+        structs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], t["E_i"].data_ptr(), t["E_j"].data_ptr(),
+                               t["op"].data_ptr(), t["op"].data_ptr(), p["dim_i"], t["rho_i"].data_ptr(), p["K_i"],
+                               t["rho_j"].data_ptr(), p["K_j"], 1.0))
+    bins = [torch.zeros(grid.nbins, dtype=torch.float64, device=dev) for _ in range(4)]
+    ctx = Context(0)
+    T, wnf = 1e-3, 0.37
+
+    def step():
+        for b in bins:
+            b.zero_()
+        ctx.fdm_bin(structs, grid, T, wnf, -1, *bins)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log("  " + ", ".join(f"{k} {v[0] / 2:.3f} ms ({v[1] // 2} launches)" for k, v in kt.items()))
+    abytes = 16.0 * nel + 8.0 * sum(p["dim_i"] + p["dim_j"] for p in pairs)
+    tms, cnt = kt["k_fdm_bin"]
+    ach = abytes / (tms / cnt * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_fdm_bin", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "note": "algorithmic bytes = 16 B per element (op1 + op2 or the rho product) + 8 B per energy"}
+    cpu = None
+    if not args.no_cpu:
+        sample = pairs
+        hs, hold = [], []
+        for p in sample:
+            arrs = {k: np.ascontiguousarray(p[k]) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+            hold.append(arrs)
+            hs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], arrs["E_i"].ctypes.data,
+                              arrs["E_j"].ctypes.data, arrs["op"].ctypes.data, arrs["op"].ctypes.data, p["dim_i"],
+                              arrs["rho_i"].ctypes.data, p["K_i"], arrs["rho_j"].ctypes.data, p["K_j"], 1.0))
+        oracle.lib().oracle_blas_threads(1)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 500:
+            oracle.fdm_bin(hs, grid, T, wnf, -1, grid.nbins)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * workloads.fdm_elements(sample) / dt / 1e6, 2), "unit": "Melements/s",
+               "cores": 1, "kind": "port",
+               "sample": f"the whole step ({len(pairs)} pairs) {reps} times, scalar term loops + serial binning, "
+                         f"{dt:.1f} s"}
+    out = {"metric": "FDM spectral weights + log binning, elements per second", "value": round(nel / ms / 1e3, 2),
+           "unit": "Melements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"one FDM step, {len(pairs)} subspace pairs, dims log-uniform [16,2000], 40 % kept, "
+                                  "1000 bins/decade over 14 decades, seed 20261206", "elements_per_half": nel},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm"], default="eig")
+    ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
+    ap.add_argument("--nblk", type=int, default=256)
+    ap.add_argument("--npairs", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the eigensolver shards by blocks across GPUs in "
+              "design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -71,3 +71,65 @@ def eig_bytes(sizes):
     """Algorithmic HBM bytes: read the upper triangle, write the eigenvectors and eigenvalues."""
     s = np.asarray(sizes, dtype=np.float64)
     return float(np.sum(8.0 * (s * (s + 1) / 2 + s * s + s)))
+
+
+def recalc_table(npairs=48, dmin=16, dmax=2000, combs=16, seed=SEED + 2):
+    """The split-precision recalc workload: npairs output blocks <I1||O||Ip> with dims log-uniform in
+    [dmin, dmax]; each subspace dimension is a random composition of `combs` ancestor dimensions (16
+    for two-channel QSZ) and every output gets one table row per ancestor (the 16-row
+    qsz-2ch-doublet tables: i1 = ip = i), i.e. the recalc_general term
+        C(I1,Ip) += f_i U_I1(:, part_i) O_i U_Ip(:, part'_i)^T,  O_i ~ N(0,1) of r_i x r'_i,
+    with U_I1, U_Ip Haar orthogonal and f_i = +-1. Returns a list of dicts."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(dmin), np.log(dmax), size=(npairs, 2))
+    dims = np.clip(np.rint(np.exp(u)), dmin, dmax).astype(np.int64)
+
+    def composition(d):
+        cuts = np.sort(rng.choice(np.arange(1, d), size=min(combs - 1, d - 1), replace=False)) if d > 1 else []
+        parts = np.diff(np.concatenate([[0], cuts, [d]])).astype(np.int64)
+        return np.concatenate([parts, np.zeros(combs - len(parts), np.int64)])
+
+    out = []
+    for dim1, dimp in dims:
+        r1, rp = composition(int(dim1)), composition(int(dimp))
+        U1, Up = haar_orthogonal(rng, int(dim1)), haar_orthogonal(rng, int(dimp))
+        Os = [rng.standard_normal((int(a), int(b))) for a, b in zip(r1, rp)]
+        f = rng.choice([-1.0, 1.0], size=combs)
+        out.append(dict(dim1=int(dim1), dimp=int(dimp), r1=r1, rp=rp, U1=U1, Up=Up, O=Os, f=f))
+    return out
+
+
+def recalc_flops(table):
+    """Algorithmic FP64 flops in the reference's order (c++/numerics_ublas.hpp:118-128):
+    T = O U_p^T (2 r rp dimp), then C += U1 T (2 dim1 r dimp), per table row."""
+    s1 = s2 = 0.0
+    for t in table:
+        for a, b in zip(t["r1"], t["rp"]):
+            if a and b:
+                s1 += 2.0 * a * b * t["dimp"]
+                s2 += 2.0 * t["dim1"] * a * t["dimp"]
+    return s1, s2
+
+
+def fdm_step(npairs=96, dmin=16, dmax=2000, keep_frac=0.4, emax=10.0, seed=SEED + 3):
+    """One FDM step's subspace pairs: dims log-uniform in [dmin, dmax], the lowest keep_frac of each
+    subspace kept, ascending energies in [0, emax), op1 = op2 Gaussian (an A-A spectral function),
+    density matrices random symmetric over the kept states."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(dmin), np.log(dmax), size=(npairs, 2))
+    dims = np.clip(np.rint(np.exp(u)), dmin, dmax).astype(np.int64)
+    pairs = []
+    for di, dj in dims:
+        Ki, Kj = int(round(keep_frac * di)), int(round(keep_frac * dj))
+        Ei, Ej = np.sort(rng.uniform(0, emax, int(di))), np.sort(rng.uniform(0, emax, int(dj)))
+        op = rng.standard_normal((int(dj), int(di)))
+        ri = rng.standard_normal((Ki, Ki))
+        rj = rng.standard_normal((Kj, Kj))
+        pairs.append(dict(dim_i=int(di), dim_j=int(dj), K_i=Ki, K_j=Kj, E_i=Ei, E_j=Ej, op=op,
+                          rho_i=(ri + ri.T) / max(Ki, 1), rho_j=(rj + rj.T) / max(Kj, 1)))
+    return pairs
+
+
+def fdm_elements(pairs):
+    """Elements evaluated per half (gt or ls): |D_i||D_j| + |D_i||K_j| + |K_i||D_j|."""
+    return sum(p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"] for p in pairs)
