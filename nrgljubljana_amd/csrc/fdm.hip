@@ -198,9 +198,7 @@ int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_g
   NRG_HIP(hipEventRecord(ctx->ev[0], st));
   if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
   if (!gitems.empty())
-    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(gitems.size() / 2)), dim3(256), 0, st,
-                                                        (const GemmDesc *)ctx->desc.as<char>(oG),
-                                                        (const int2 *)ctx->desc.as<char>(oGI)));
+    gemm_launch(ctx, st, (const GemmDesc *)ctx->desc.as<char>(oG), ctx->desc.as<int>(oGI), gitems.data(), gitems.size() / 2);
   NRG_HIP(hipEventRecord(ctx->ev[1], st));
   FdmArgs fa;
   fa.pairs = (const FdmPairDev *)ctx->desc.as<char>(oPd);

@@ -1,11 +1,13 @@
-// gemm.hip - batched FP64 products on v_mfma_f64_16x16x4f64 (see gemm.hpp).
+// gemm.hip - batched products on the matrix cores (see gemm.hpp).
 //
-// A 256-thread workgroup computes a 64x64 tile of C; wave w owns the 32x32 quarter
-// (w % 2, w / 2) as 2x2 MFMA tiles. K advances in chunks of 32 staged through LDS; the next
-// chunk is fetched into registers while the matrix cores work on the current one.
-// The MFMA computes D = op(B)^T op(A)^T = C^T so that in the result layout of the
-// instruction (lane l, register i holds D[l/16 + 4 i][l % 16]) consecutive lanes hold
-// consecutive rows of the column-major C, and the stores coalesce.
+// A 256-thread workgroup computes one TM x TM tile of C: TM = 64 for small products (wave w owns
+// the 32 x 32 quarter (w % 2, w / 2) as 2 x 2 MFMA tiles) and TM = 128 when M, N >= 128 (each wave
+// 64 x 64 as 4 x 4 MFMA tiles: twice the flops per byte staged and per LDS fragment read). K
+// advances in chunks (32 for the small tile, 16 for the big one) staged through LDS; the next chunk is fetched into registers while the
+// matrix cores work on the current one. The MFMA computes D = op(B)^T op(A)^T = C^T so that in the
+// result layout of the instruction (FP64: lane l, register i holds D[l/16 + 4 i][l % 16]; FP32:
+// D[4 (l/16) + i][l % 16]) consecutive lanes hold consecutive rows of the column-major C, and the
+// stores coalesce.
 #include "gemm.hpp"
 
 namespace nrg {
@@ -24,7 +26,6 @@ template <typename T> struct Acc;
 template <> struct Acc<double> { typedef v4d type; };
 template <> struct Acc<float> { typedef v4f type; };
 
-// FP64: lane l, register r holds D[l/16 + 4 r][l % 16]; FP32: D[4 (l/16) + r][l % 16]
 template <typename T> __device__ __forceinline__ int out_row(int lane, int r);
 template <> __device__ __forceinline__ int out_row<double>(int lane, int r) { return (lane >> 4) + 4 * r; }
 template <> __device__ __forceinline__ int out_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
@@ -32,66 +33,67 @@ template <> __device__ __forceinline__ int out_row<float>(int lane, int r) { ret
 __device__ __forceinline__ v4d mfma(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ v4f mfma(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
-constexpr int KC = 32;   // K chunk staged through LDS
-
-template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmDescT<T> G, const int2 it) {
-  const int tm = (G.M + GEMM_TILE - 1) / GEMM_TILE;
-  const int m0 = (it.y % tm) * GEMM_TILE, n0 = (it.y / tm) * GEMM_TILE;
-  __shared__ T As[KC][GEMM_TILE + 1];   // op(A)(m0 + m, k0 + k) at As[k][m]
-  __shared__ T Bs[KC][GEMM_TILE + 1];   // op(B)(k0 + k, n0 + n) at Bs[k][n]
+// TM x TM tile of C with K chunks of KC staged through LDS
+template <typename T, int TM, int KC>
+__device__ __forceinline__ void gemm_tile(const GemmDescT<T> &G, int tile, T *lds) {
+  constexpr int LD = TM + 1;          // LDS row stride
+  constexpr int NF = TM / 32;         // MFMA tiles per wave and dimension
+  constexpr int PT = TM * KC / 256;   // elements of each operand staged per thread and chunk
+  const int tm = (G.M + TM - 1) / TM;
+  const int m0 = (tile % tm) * TM, n0 = (tile / tm) * TM;
+  T *As = lds;             // op(A)(m0 + m, k0 + k) at As[k * LD + m]
+  T *Bs = lds + KC * LD;   // op(B)(k0 + k, n0 + n) at Bs[k * LD + n]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
-  typename Acc<T>::type acc[2][2];
+  const int wm = (wv & 1) * (TM / 2), wn = (wv >> 1) * (TM / 2);
+  typename Acc<T>::type acc[NF][NF];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < NF; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (typename Acc<T>::type){0, 0, 0, 0};
-  // each thread stages 8 elements of op(A) and 8 of op(B) per chunk
-  auto fetch = [&](const T *A, const T *B, int lda, int ldb, int tA, int tB, int Kp, int kb, T sc, T (&ra)[8],
-                   T (&rb)[8]) {
+    for (int j = 0; j < NF; j++) acc[i][j] = (typename Acc<T>::type){0, 0, 0, 0};
+  auto fetch = [&](const T *A, const T *B, int lda, int ldb, int tA, int tB, int Kp, int kb, T sc, T (&ra)[PT],
+                   T (&rb)[PT]) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < PT; q++) {
       const int idx = tid + 256 * q;
       if (tA) {   // contiguous along k
-        const int k = idx & (KC - 1), m = idx / KC;
+        const int k = idx % KC, m = idx / KC;
         ra[q] = ld_op(A, lda, 1, m0 + m, kb + k, G.M, Kp);
       } else {    // contiguous along m
-        const int m = idx & 63, k = idx >> 6;
+        const int m = idx % TM, k = idx / TM;
         ra[q] = ld_op(A, lda, 0, m0 + m, kb + k, G.M, Kp);
       }
       if (tB) {   // contiguous along n
-        const int n = idx & 63, k = idx >> 6;
+        const int n = idx % TM, k = idx / TM;
         rb[q] = sc * ld_op(B, ldb, 1, kb + k, n0 + n, Kp, G.N);
       } else {    // contiguous along k
-        const int k = idx & (KC - 1), n = idx / KC;
+        const int k = idx % KC, n = idx / KC;
         rb[q] = sc * ld_op(B, ldb, 0, kb + k, n0 + n, Kp, G.N);
       }
     }
   };
-  auto stash = [&](int tA, int tB, const T (&ra)[8], const T (&rb)[8]) {
+  auto stash = [&](int tA, int tB, const T (&ra)[PT], const T (&rb)[PT]) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < PT; q++) {
       const int idx = tid + 256 * q;
-      if (tA) As[idx & (KC - 1)][idx / KC] = ra[q];
-      else As[idx >> 6][idx & 63] = ra[q];
-      if (tB) Bs[idx >> 6][idx & 63] = rb[q];
-      else Bs[idx & (KC - 1)][idx / KC] = rb[q];
+      if (tA) As[(idx % KC) * LD + idx / KC] = ra[q];
+      else As[(idx / TM) * LD + idx % TM] = ra[q];
+      if (tB) Bs[(idx / TM) * LD + idx % TM] = rb[q];
+      else Bs[(idx % KC) * LD + idx / KC] = rb[q];
     }
   };
   auto compute = [&]() {
 #pragma unroll
     for (int kk = 0; kk < KC / 4; kk++) {
       const int k = kk * 4 + (lane >> 4);
-      T fa[2], fb[2];
+      T fa[NF], fb[NF];
 #pragma unroll
-      for (int j = 0; j < 2; j++) fa[j] = Bs[k][wn + j * 16 + (lane & 15)];   // MFMA A operand: op(B)^T
+      for (int j = 0; j < NF; j++) fa[j] = Bs[k * LD + wn + j * 16 + (lane & 15)];   // MFMA A operand: op(B)^T
 #pragma unroll
-      for (int i = 0; i < 2; i++) fb[i] = As[k][wm + i * 16 + (lane & 15)];   // MFMA B operand: op(A)^T
+      for (int i = 0; i < NF; i++) fb[i] = As[k * LD + wm + i * 16 + (lane & 15)];   // MFMA B operand: op(A)^T
 #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int i = 0; i < NF; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = mfma(fa[j], fb[i], acc[i][j]);
+        for (int j = 0; j < NF; j++) acc[i][j] = mfma(fa[j], fb[i], acc[i][j]);
     }
   };
   // K-parts: the inline pair (A, B) and (A2, B2), or the descriptor's part list
@@ -114,7 +116,7 @@ __device__ __forceinline__ void gemm_tile(const GemmDescT<T> G, const int2 it) {
   part_of(0, A, B, lda, ldb, tA, tB, Kp, sc);
   while (p < np && Kp <= 0) { if (++p < np) part_of(p, A, B, lda, ldb, tA, tB, Kp, sc); }
   if (p < np) {
-    T ra[8], rb[8];
+    T ra[PT], rb[PT];
     fetch(A, B, lda, ldb, tA, tB, Kp, 0, sc, ra, rb);
     int ctA = tA, ctB = tB;
     for (;;) {
@@ -137,9 +139,9 @@ __device__ __forceinline__ void gemm_tile(const GemmDescT<T> G, const int2 it) {
   }
   // acc[i][j] holds D = C^T for rows n0+wn+16j+out_row(lane, r) (of D) and column m0+wm+16i+(lane&15)
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < NF; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < NF; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = m0 + wm + i * 16 + (lane & 15);
@@ -153,13 +155,55 @@ __device__ __forceinline__ void gemm_tile(const GemmDescT<T> G, const int2 it) {
 }
 
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc *descs, const int2 *items) {
+  __shared__ double lds[2 * 32 * (GEMM_TILE + 1)];
   const int2 it = items[blockIdx.x];
-  gemm_tile<double>(descs[it.x], it);
+  gemm_tile<double, GEMM_TILE, 32>(descs[it.x], it.y, lds);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_f32(const GemmDesc32 *descs, const int2 *items) {
+  __shared__ float lds[2 * 32 * (GEMM_TILE + 1)];
   const int2 it = items[blockIdx.x];
-  gemm_tile<float>(descs[it.x], it);
+  gemm_tile<float, GEMM_TILE, 32>(descs[it.x], it.y, lds);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_f64_big(const GemmDesc *descs, const int2 *items) {
+  __shared__ double lds[2 * 16 * (GEMM_BIG + 1)];
+  const int2 it = items[blockIdx.x];
+  gemm_tile<double, GEMM_BIG, 16>(descs[it.x], it.y & ~GEMM_BIG_FLAG, lds);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_f32_big(const GemmDesc32 *descs, const int2 *items) {
+  __shared__ float lds[2 * 16 * (GEMM_BIG + 1)];
+  const int2 it = items[blockIdx.x];
+  gemm_tile<float, GEMM_BIG, 16>(descs[it.x], it.y & ~GEMM_BIG_FLAG, lds);
+}
+
+static size_t leading_big(const int *items_host, size_t count) {
+  size_t nb = 0;
+  while (nb < count && (items_host[2 * nb + 1] & GEMM_BIG_FLAG)) nb++;
+  return nb;
+}
+
+void gemm_launch(nrg_ctx *ctx, hipStream_t st, const GemmDesc *descs, const int *items_dev, const int *items_host,
+                 size_t count) {
+  const size_t nb = leading_big(items_host, count);
+  if (nb)
+    NRG_TIMED(ctx, st, "k_gemm_f64_big", hipLaunchKernelGGL(k_gemm_f64_big, dim3((unsigned)nb), dim3(256), 0, st, descs,
+                                                            (const int2 *)items_dev));
+  if (count > nb)
+    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(count - nb)), dim3(256), 0, st, descs,
+                                                        (const int2 *)items_dev + nb));
+}
+
+void gemm_launch(nrg_ctx *ctx, hipStream_t st, const GemmDesc32 *descs, const int *items_dev, const int *items_host,
+                 size_t count) {
+  const size_t nb = leading_big(items_host, count);
+  if (nb)
+    NRG_TIMED(ctx, st, "k_gemm_f32_big", hipLaunchKernelGGL(k_gemm_f32_big, dim3((unsigned)nb), dim3(256), 0, st, descs,
+                                                            (const int2 *)items_dev));
+  if (count > nb)
+    NRG_TIMED(ctx, st, "k_gemm_f32", hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)(count - nb)), dim3(256), 0, st, descs,
+                                                        (const int2 *)items_dev + nb));
 }
 
 }  // namespace nrg

@@ -31,24 +31,45 @@ using GemmDesc = GemmDescT<double>;
 using GemmPart = GemmPartT<double>;
 using GemmDesc32 = GemmDescT<float>;
 
-constexpr int GEMM_TILE = 64;
+constexpr int GEMM_TILE = 64;         // tile of C per workgroup for small products
+constexpr int GEMM_BIG = 128;         // tile for products with M, N >= GEMM_BIG
+constexpr int GEMM_BIG_FLAG = 1 << 30;   // marks a work item's tile index as a GEMM_BIG tile
+
+// Big tiles pay off when the tile has enough K to amortise its staging: M, N >= 128 and total
+// K >= 256 (descriptors with a part list carry their total K in K as a hint).
+template <typename T>
+inline bool gemm_is_big(const GemmDescT<T> &g) {
+  const int ktot = g.nparts > 0 ? g.K : g.K + (g.K2 > 0 ? g.K2 : 0);
+  return g.M >= GEMM_BIG && g.N >= GEMM_BIG && ktot >= 256;
+}
 
 template <typename T>
 inline int gemm_tiles(const GemmDescT<T> &g) {
-  return ((g.M + GEMM_TILE - 1) / GEMM_TILE) * ((g.N + GEMM_TILE - 1) / GEMM_TILE);
+  const int t = gemm_is_big(g) ? GEMM_BIG : GEMM_TILE;
+  return ((g.M + t - 1) / t) * ((g.N + t - 1) / t);
 }
 
-// appends (desc index, tile) items of descriptors [d0, d1) to `items`
+// appends (desc index, tile) items of descriptors [d0, d1) to `items`: first every big tile (tile
+// index with GEMM_BIG_FLAG), then every small one, so that gemm_launch can split the list
 template <typename T>
 inline void gemm_items(const std::vector<GemmDescT<T>> &d, size_t d0, size_t d1, std::vector<int> &items) {
-  for (size_t i = d0; i < d1; i++) {
-    const int t = gemm_tiles(d[i]);
-    for (int k = 0; k < t; k++) {
-      items.push_back((int)i);
-      items.push_back(k);
+  for (int big = 1; big >= 0; big--)
+    for (size_t i = d0; i < d1; i++) {
+      if ((int)gemm_is_big(d[i]) != big) continue;
+      const int t = gemm_tiles(d[i]);
+      for (int k = 0; k < t; k++) {
+        items.push_back((int)i);
+        items.push_back(big ? (k | GEMM_BIG_FLAG) : k);
+      }
     }
-  }
 }
+
+// Launches the `count` work items of a gemm_items list: the leading big tiles on k_gemm_*_big,
+// the rest on k_gemm_* (items_host: the same list in host memory).
+void gemm_launch(nrg_ctx *ctx, hipStream_t st, const GemmDesc *descs, const int *items_dev, const int *items_host,
+                 size_t count);
+void gemm_launch(nrg_ctx *ctx, hipStream_t st, const GemmDesc32 *descs, const int *items_dev, const int *items_host,
+                 size_t count);
 
 // The two stages of a grouped triple-product table (nrg_gemm3_grouped): gd[0, nd1) form the
 // intermediates, gd[nd1, ...) accumulate into each distinct C with K-parts
@@ -63,5 +84,7 @@ int gemm3_plan(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, Gemm3Plan 
 
 __global__ void k_gemm_f64(const GemmDesc *descs, const int2 *items);
 __global__ void k_gemm_f32(const GemmDesc32 *descs, const int2 *items);
+__global__ void k_gemm_f64_big(const GemmDesc *descs, const int2 *items);
+__global__ void k_gemm_f32_big(const GemmDesc32 *descs, const int2 *items);
 
 }  // namespace nrg

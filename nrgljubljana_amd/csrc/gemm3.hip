@@ -100,6 +100,7 @@ int gemm3_plan(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, Gemm3Plan 
     GemmDesc d{};
     d.C = C; d.ldc = (int)t0.ldc; d.M = (int)t0.n; d.N = (int)t0.m;
     d.alpha = 1.0; d.beta = 1.0; d.nparts = (int)(pl.parts.size() - p0);
+    for (size_t q = p0; q < pl.parts.size(); q++) d.K += pl.parts[q].K;   // total K: a tile-shape hint only
     pl.gd.push_back(d);
     pl.prange.push_back({p0, pl.parts.size()});
   }
@@ -135,13 +136,9 @@ int gemm3_fp64(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream_
   NRG_HIP(hipEventRecord(ctx->ev[0], st));
   if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
   const GemmDesc *dd = (const GemmDesc *)ctx->desc.as<char>(oD);
-  if (!items1.empty())
-    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(items1.size() / 2)), dim3(256), 0, st, dd,
-                                                        (const int2 *)ctx->desc.as<char>(oI1)));
+  if (!items1.empty()) gemm_launch(ctx, st, dd, ctx->desc.as<int>(oI1), items1.data(), items1.size() / 2);
   NRG_HIP(hipEventRecord(ctx->ev[1], st));
-  if (!items2.empty())
-    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(items2.size() / 2)), dim3(256), 0, st, dd,
-                                                        (const int2 *)ctx->desc.as<char>(oI2)));
+  if (!items2.empty()) gemm_launch(ctx, st, dd, ctx->desc.as<int>(oI2), items2.data(), items2.size() / 2);
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[2], st));
   NRG_HIP(hipStreamSynchronize(st));

@@ -1810,14 +1810,11 @@ struct SyevLaunch {
 };
 
 template <typename T>
-static void launch_gemm(nrg_ctx *ctx, hipStream_t st, const void *descs, const int *items, size_t count) {
+static void launch_gemm(nrg_ctx *ctx, hipStream_t st, const void *descs, const int *items, const int *items_host,
+                        size_t count) {
   if (!count) return;
-  if constexpr (sizeof(T) == 8)
-    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)count), dim3(256), 0, st,
-                                                        (const GemmDesc *)descs, (const int2 *)items));
-  else
-    NRG_TIMED(ctx, st, "k_gemm_f32", hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)count), dim3(256), 0, st,
-                                                        (const GemmDesc32 *)descs, (const int2 *)items));
+  if constexpr (sizeof(T) == 8) gemm_launch(ctx, st, (const GemmDesc *)descs, items, items_host, count);
+  else gemm_launch(ctx, st, (const GemmDesc32 *)descs, items, items_host, count);
 }
 
 // Launch sequence of the reduction + D&C + back-transformation for a typed work matrix.
@@ -1825,6 +1822,7 @@ template <typename T>
 static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<double> &aD, const SyevLaunch &L,
                        hipStream_t st, bool zt_is_z0, int phase0) {
   const int *I = L.dI;
+  const int *H = pl.ibuf.data();   // host copy of the work lists (GEMM lists are split by tile shape)
   auto grid = [](size_t c) { return dim3((unsigned)c); };
   // 1. symmetrisation
   if (pl.symm_cnt)
@@ -1843,7 +1841,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
         NRG_TIMED(ctx, st, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, st, aT,
                                                             I + c.symv_off, I + c.tpre_off, (int)c.nact, p, c.jl));
     }
-    launch_gemm<T>(ctx, st, L.dGT, I + pl.s2_items[p], pl.s2_cnt[p]);
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.s2_items[p], H + pl.s2_items[p], pl.s2_cnt[p]);
   }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 1], st));
@@ -1876,9 +1874,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
                                                            mi, ch, depth, (const MergeState *)L.mst));
     NRG_TIMED(ctx, st, "k_dc_fill", hipLaunchKernelGGL(k_dc_fill, grid(pl.fc_cnt[depth]), dim3(256), lds2, st, aD, mi, fc,
                                                        depth, (const MergeState *)L.mst));
-    if (pl.gemm_cnt[depth])
-      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, grid(pl.gemm_cnt[depth]), dim3(256), 0, st, L.dG,
-                         (const int2 *)(I + pl.gemm_off[depth])));
+    if (pl.gemm_cnt[depth]) gemm_launch(ctx, st, L.dG, I + pl.gemm_off[depth], H + pl.gemm_off[depth], pl.gemm_cnt[depth]);
   }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 2], st));
@@ -1889,13 +1885,13 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   }
   if (pl.ym_cnt)
     NRG_TIMED(ctx, st, "k_bt_ymat", hipLaunchKernelGGL(k_bt_ymat<T>, grid(pl.ym_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.ym_off)));
-  launch_gemm<T>(ctx, st, L.dGT, I + pl.g_items, pl.g_cnt);
+  launch_gemm<T>(ctx, st, L.dGT, I + pl.g_items, H + pl.g_items, pl.g_cnt);
   if (pl.tb_cnt)
     NRG_TIMED(ctx, st, "k_bt_tbig", hipLaunchKernelGGL(k_bt_tbig<T>, grid(pl.tb_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.tb_off), L.toff));
-  launch_gemm<T>(ctx, st, L.dGT, I + pl.yh_items, pl.yh_cnt);
+  launch_gemm<T>(ctx, st, L.dGT, I + pl.yh_items, H + pl.yh_items, pl.yh_cnt);
   for (int step = 0; step < pl.Smax; step++) {
-    launch_gemm<T>(ctx, st, L.dGT, I + pl.w_items[step], pl.w_cnt[step]);
-    launch_gemm<T>(ctx, st, L.dGT, I + pl.u_items[step], pl.u_cnt[step]);
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.w_items[step], H + pl.w_items[step], pl.w_cnt[step]);
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.u_items[step], H + pl.u_items[step], pl.u_cnt[step]);
   }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 3], st));
@@ -2152,15 +2148,12 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
       NRG_TIMED(ctx, st, "k_ref_eye", hipLaunchKernelGGL(k_ref_eye, dim3((unsigned)rc_cnt), dim3(256), 0, st, ra, rci));
     NRG_HIP(hipMemsetAsync(ra.acc, 0, sizeof(double) * 3 * nblk, st));
     for (int ph = 0; ph < 2; ph++)
-      NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)rg_cnt[dir][ph]), dim3(256), 0, st, dR,
-                                                          (const int2 *)(I + rg_off[dir][ph])));
+      gemm_launch(ctx, st, dR, I + rg_off[dir][ph], pl.ibuf.data() + rg_off[dir][ph], rg_cnt[dir][ph]);
     NRG_TIMED(ctx, st, "k_ref_lam", hipLaunchKernelGGL(k_ref_lam, dim3((unsigned)nblk), dim3(256), 0, st, ra, I + pl.all_off));
     NRG_TIMED(ctx, st, "k_ref_norms", hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
     NRG_TIMED(ctx, st, "k_ref_E", hipLaunchKernelGGL(k_ref_E, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
-    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)rg_cnt[dir][3]), dim3(256), 0, st, dR,
-                                                        (const int2 *)(I + rg_off[dir][3])));
-    NRG_TIMED(ctx, st, "k_gemm_f64", hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)rg_cnt[dir][2]), dim3(256), 0, st, dR,
-                                                        (const int2 *)(I + rg_off[dir][2])));
+    gemm_launch(ctx, st, dR, I + rg_off[dir][3], pl.ibuf.data() + rg_off[dir][3], rg_cnt[dir][3]);
+    gemm_launch(ctx, st, dR, I + rg_off[dir][2], pl.ibuf.data() + rg_off[dir][2], rg_cnt[dir][2]);
     NRG_TIMED(ctx, st, "k_ref_clusters", hipLaunchKernelGGL(k_ref_clusters, dim3((unsigned)nblk), dim3(256), 0, st, ra,
                                                             I + pl.all_off));
   }
