@@ -527,7 +527,7 @@ __global__ __launch_bounds__(256) void k_dc_prep(SyevArgs<double> a, const int *
 // 3b. divide and conquer merges. Each merge of two sorted spectra with the rank-one tear
 //     rho z z^T (dlaed1) runs as five launches over all merges of one tree depth:
 //       k_dc_deflate   merge order, deflation (dlaed2), one workgroup per merge
-//       k_dc_roots     secular equation roots (dlaed4), 256 roots per workgroup
+//       k_dc_roots     secular equation roots (dlaed4), 16 lanes per root, 16 roots per workgroup
 //       k_dc_zhat      Loewner recomputation of z (Gu-Eisenstat, dlaed3)
 //       k_dc_normrank  eigenvector norms and the ascending order of all eigenvalues
 //       k_dc_fill      eigenvector matrix S of the merge (+ undoing the deflation rotations),
@@ -564,7 +564,18 @@ __device__ __forceinline__ int *isa(const SyevArgs<double> &a, const BlkDesc &bd
 // The origin is the nearer pole o (lambda = d_o + tau); the first iterate solves the model with
 // the two adjacent poles exact and the rest frozen; then Newton on the function with both
 // adjacent poles multiplied out, safeguarded by bisection.
-__device__ void secular_root(int j, int k, const double *d, const double *z, double rho, double zz, int &org,
+// A group of SL consecutive lanes solves one root: every sum over the poles is split across the
+// group (lane `sub` takes i = sub mod SL) and combined with an xor butterfly, which gives every
+// lane of the group the same bits (the pairwise additions commute), so the group's lanes take the
+// same iterations and branches.
+constexpr int SL = 16;   // lanes per secular root
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = SL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+__device__ void secular_root(int j, int k, const double *d, const double *z, double rho, double zz, int sub, int &org,
                              double &tau_out) {
   const double EPS = 2.220446049250313e-16;
   int o, o2;
@@ -572,10 +583,11 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
   if (j < k - 1) {
     const double g = d[j + 1] - d[j], mid = 0.5 * g;
     double R0 = 0.0;
-    for (int i = 0; i < k; i++) {
+    for (int i = sub; i < k; i += SL) {
       if (i == j || i == j + 1) continue;
       R0 += rho * z[i] * z[i] / ((d[i] - d[j]) - mid);
     }
+    R0 = group_sum(R0);
     const double f = 1.0 + R0 - rho * z[j] * z[j] / mid + rho * z[j + 1] * z[j + 1] / (g - mid);
     if (f >= 0) { o = j; o2 = j + 1; lo = 0; hi = mid; }
     else { o = j + 1; o2 = j; lo = -mid; hi = 0; }
@@ -595,7 +607,8 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
     za = rho * z[o] * z[o];
     const double t0 = 0.5 * hi;
     double R0 = 0.0;
-    for (int i = 0; i < k - 1; i++) R0 += rho * z[i] * z[i] / ((d[i] - d[o]) - t0);
+    for (int i = sub; i < k - 1; i += SL) R0 += rho * z[i] * z[i] / ((d[i] - d[o]) - t0);
+    R0 = group_sum(R0);
     t = t0;
     if (1.0 + R0 > 0) {
       const double tc = za / (1.0 + R0);
@@ -607,11 +620,11 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
     // the two adjacent poles are masked out (their denominators -t and d2 - t are nonzero);
     // independent divisions in flight hide their latency
     double R = 0, Rp = 0, Ra = 0, R2 = 0, Rp2 = 0, Ra2 = 0;
-    int i = 0;
-    for (; i + 1 < k; i += 2) {
-      const double q0 = 1.0 / ((d[i] - dorg) - t), q1 = 1.0 / ((d[i + 1] - dorg) - t);
+    int i = sub;
+    for (; i + SL < k; i += 2 * SL) {
+      const double q0 = 1.0 / ((d[i] - dorg) - t), q1 = 1.0 / ((d[i + SL] - dorg) - t);
       const double t0 = (i == o || i == o2) ? 0.0 : rho * z[i] * z[i] * q0;
-      const double t1 = (i + 1 == o || i + 1 == o2) ? 0.0 : rho * z[i + 1] * z[i + 1] * q1;
+      const double t1 = (i + SL == o || i + SL == o2) ? 0.0 : rho * z[i + SL] * z[i + SL] * q1;
       R += t0; Rp += t0 * q0; Ra += fabs(t0);
       R2 += t1; Rp2 += t1 * q1; Ra2 += fabs(t1);
     }
@@ -620,7 +633,9 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
       const double t0 = (i == o || i == o2) ? 0.0 : rho * z[i] * z[i] * q0;
       R += t0; Rp += t0 * q0; Ra += fabs(t0);
     }
-    R += R2; Rp += Rp2; Ra += Ra2;
+    R = group_sum(R + R2);
+    Rp = group_sum(Rp + Rp2);
+    Ra = group_sum(Ra + Ra2);
     double f, err, gv, gp;
     const double pa = -za / t;
     if (o2 >= 0) {
@@ -749,12 +764,13 @@ __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const
   for (int p = tid; p < m; p += NT_DEF) ds[p] = dsl[p];
 }
 
-// chunk items: .x = merge index within the depth's merge list, .y = chunk of RCH roots
-constexpr int RCH = 64;
-__global__ __launch_bounds__(RCH) void k_dc_roots(SyevArgs<double> a, const int2 *mitems, const int2 *citems,
+// chunk items: .x = merge index within the depth's merge list, .y = chunk of RCH roots;
+// a 256-thread workgroup solves RCH = 256 / SL roots
+constexpr int RCH = 256 / SL;
+__global__ __launch_bounds__(256) void k_dc_roots(SyevArgs<double> a, const int2 *mitems, const int2 *citems,
                                                   int depth, const MergeState *mst) {
   extern __shared__ double dyn[];
-  __shared__ double red[RCH / WAVE];
+  __shared__ double red[256 / WAVE];
   const int2 ci = citems[blockIdx.x];
   const int2 it = mitems[ci.x];
   const BlkDesc bd = a.bd[it.x];
@@ -765,19 +781,21 @@ __global__ __launch_bounds__(RCH) void k_dc_roots(SyevArgs<double> a, const int2
   double *dl = dyn, *zl = dyn + g.m;
   const double *dnd = dsa(a, bd, DS_DND, g.a0), *znd = dsa(a, bd, DS_ZND, g.a0);
   double zz = 0;
-  for (int i = threadIdx.x; i < k; i += RCH) {
+  for (int i = threadIdx.x; i < k; i += 256) {
     dl[i] = dnd[i];
     zl[i] = znd[i];
     zz += znd[i] * znd[i];
   }
-  zz = block_sum<RCH>(zz, red);
-  const int j = ci.y * RCH + threadIdx.x;
-  if (j < k) {
+  zz = block_sum<256>(zz, red);
+  const int j = ci.y * RCH + threadIdx.x / SL, sub = threadIdx.x % SL;
+  if (j < k) {   // uniform within each group of SL lanes
     int o; double t;
-    secular_root(j, k, dl, zl, ms.rho, zz, o, t);
-    isa(a, bd, IS_RO, g.a0)[j] = o;
-    dsa(a, bd, DS_RT, g.a0)[j] = t;
-    dsa(a, bd, DS_DORG, g.a0)[j] = dl[o];
+    secular_root(j, k, dl, zl, ms.rho, zz, sub, o, t);
+    if (sub == 0) {
+      isa(a, bd, IS_RO, g.a0)[j] = o;
+      dsa(a, bd, DS_RT, g.a0)[j] = t;
+      dsa(a, bd, DS_DORG, g.a0)[j] = dl[o];
+    }
   }
 }
 
@@ -1866,7 +1884,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     const int2 *fc = (const int2 *)(I + pl.fc_off[depth]);
     NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate, grid(pl.mrg_cnt[depth]), dim3(NT_DEF), lds2, st, aD,
                                                           mi, depth, L.mst));
-    NRG_TIMED(ctx, st, "k_dc_roots", hipLaunchKernelGGL(k_dc_roots, grid(pl.rc_cnt[depth]), dim3(RCH), lds2, st, aD, mi,
+    NRG_TIMED(ctx, st, "k_dc_roots", hipLaunchKernelGGL(k_dc_roots, grid(pl.rc_cnt[depth]), dim3(256), lds2, st, aD, mi,
                                                         (const int2 *)(I + pl.rc_off[depth]), depth, (const MergeState *)L.mst));
     NRG_TIMED(ctx, st, "k_dc_zhat", hipLaunchKernelGGL(k_dc_zhat, grid(pl.ch_cnt[depth]), dim3(256), lds3, st, aD, mi, ch,
                                                        depth, (const MergeState *)L.mst));
