@@ -82,6 +82,18 @@ def reduction_bytes(sizes, nb=32):
     return {"k_col_prep": (prep_b, prep_l), "k_col_symv": (symv_b, symv_l)}
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs (profiles/), when
+    they were measured on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic_per_launch.json")
+    if args.mode != "fp64" or args.nblk != 256 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return k["bytes"] if k else None
+
+
 def cpu_baseline(sizes, off_H, arena, off_w, flops, budget_s=30.0):
     """The oracle (C restatement: OpenMP over blocks, largest first, LAPACK dsyev per block) timed on
     this host's cores in the two thread layouts of the reference; the better one is reported."""
@@ -176,7 +188,7 @@ def bench_eig(args):
         avg_s = tms / cnt * 1e-3
         ach = (tot_bytes / nl) / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
                     "bytes_per_launch": round(tot_bytes / nl), "launches_per_step": nl,
                     "avg_launch_us": round(avg_s * 1e6, 3)}
     elif dom is not None:
