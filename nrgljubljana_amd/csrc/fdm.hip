@@ -11,7 +11,8 @@
 // each at energy E_j - E_i, times the pair factor. Launch 1 forms the two products (only the
 // sub-blocks the three ranges use) on the FP64 matrix cores; launch 2 evaluates every element of
 // every pair in 64 x 64 tiles (Boltzmann factors of the tile's states in LDS, one log10 per element
-// shared by both halves) and scatters into the bins with FP64 atomics.
+// shared by both halves) and scatters into replicated bins with FP64 atomics, which one more launch
+// adds into the caller's arrays.
 #include "gemm.hpp"
 #include <cmath>
 #include <cstring>
@@ -46,9 +47,39 @@ __device__ __forceinline__ void bin_add(const FdmArgs &a, double *bins, double e
   atomicAdd(bins + idx + 1, rem * w);
 }
 
+// The scatter goes to FDM_COPIES replicas of the four histograms (workgroup w adds into replica
+// w mod FDM_COPIES): hot bins receive thousands of additions per step, and replicas spread the
+// read-modify-writes on one address over many addresses. k_fdm_reduce then adds the replicas into
+// the caller's arrays.
+constexpr int FDM_COPIES = 32;
+
+__global__ __launch_bounds__(256) void k_fdm_zero(double *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0;
+}
+
+// replica c of histogram h (0 gt_pos, 1 gt_neg, 2 ls_pos, 3 ls_neg) at priv + (c * 4 + h) * nbins
+__global__ __launch_bounds__(256) void k_fdm_reduce(const double *priv, int64_t nbins, double *gt_pos, double *gt_neg,
+                                                    double *ls_pos, double *ls_neg) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int h = blockIdx.y;
+  if (b >= nbins) return;
+  double s = 0.0;
+#pragma unroll 8
+  for (int c = 0; c < FDM_COPIES; c++) s += priv[((size_t)c * 4 + h) * nbins + b];
+  double *out = h == 0 ? gt_pos : h == 1 ? gt_neg : h == 2 ? ls_pos : ls_neg;
+  out[b] += s;
+}
+
 __global__ __launch_bounds__(256) void k_fdm_bin(FdmArgs a, const int2 *items) {
   const int2 it = items[blockIdx.x];
   const FdmPairDev P = a.pairs[it.x];
+  {
+    const size_t c = (size_t)(blockIdx.x % FDM_COPIES) * 4 * a.nbins;
+    a.gt_pos += c;
+    a.gt_neg += c + a.nbins;
+    a.ls_pos += c + 2 * a.nbins;
+    a.ls_neg += c + 3 * a.nbins;
+  }
   const int ti = (int)((P.dim_i + 63) / 64);
   const int i0 = (it.y % ti) * 64, j0 = (it.y / ti) * 64;
   __shared__ double Ei[64], Ej[64], Bi[64], Bj[64];
@@ -114,6 +145,10 @@ int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_g
     set_error("nrg_fdm_bin: invalid arguments");
     return NRG_E_ARG;
   }
+  if (grid->nbins > (1 << 20)) {   // the replicated histograms take 1 KiB per bin
+    set_error("nrg_fdm_bin: more than 2^20 bins");
+    return NRG_E_ARG;
+  }
   for (int p = 0; p < npairs; p++) {
     const nrg_fdm_pair &q = pairs[p];
     if (q.dim_i < 0 || q.dim_j < 0 || q.K_i < 0 || q.K_j < 0 || q.K_i > q.dim_i || q.K_j > q.dim_j ||
@@ -142,6 +177,7 @@ int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_g
     if (q.K_j > 0 && q.dim_i > q.K_i) { o_ro[p] = tot; tot += q.K_j * (q.dim_i - q.K_i); }
   }
   if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
+  if (int r = ctx->ws.ensure(sizeof(double) * (size_t)FDM_COPIES * 4 * (size_t)grid->nbins)) return r;
   double *prod = ctx->ws2.as<double>();
   std::vector<GemmDesc> gd;
   std::vector<FdmPairDev> pd(npairs);
@@ -205,10 +241,18 @@ int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_g
   fa.T = T; fa.wnf = wnf; fa.msign = -(double)sign;
   fa.log10emin = grid->log10emin; fa.bpd = grid->bins_per_decade; fa.discard = grid->discard_immediately;
   fa.nbins = grid->nbins;
-  fa.gt_pos = gt_pos; fa.gt_neg = gt_neg; fa.ls_pos = ls_pos; fa.ls_neg = ls_neg;
-  if (!items.empty())
+  double *priv = ctx->ws.as<double>();   // FDM_COPIES x 4 histograms
+  fa.gt_pos = priv; fa.gt_neg = priv; fa.ls_pos = priv; fa.ls_neg = priv;   // offset per replica in the kernel
+  if (!items.empty()) {
+    const size_t np = (size_t)FDM_COPIES * 4 * grid->nbins;
+    NRG_TIMED(ctx, st, "k_fdm_zero", hipLaunchKernelGGL(k_fdm_zero, dim3((unsigned)std::min<size_t>((np + 255) / 256, 4096)),
+                                                        dim3(256), 0, st, priv, np));
     NRG_TIMED(ctx, st, "k_fdm_bin", hipLaunchKernelGGL(k_fdm_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st, fa,
                                                        (const int2 *)ctx->desc.as<char>(oI)));
+    NRG_TIMED(ctx, st, "k_fdm_reduce", hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((grid->nbins + 255) / 256), 4),
+                                                          dim3(256), 0, st, priv, (int64_t)grid->nbins, gt_pos, gt_neg,
+                                                          ls_pos, ls_neg));
+  }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[2], st));
   NRG_HIP(hipStreamSynchronize(st));
