@@ -145,6 +145,19 @@ int nrg_fdm_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_b
                 double T, double wnf, int sign, double *gt_pos, double *gt_neg, double *ls_pos,
                 double *ls_neg, void *stream);
 
+/*
+ * Broadening of a binned spectrum onto an energy mesh: replaces SpectrumRealFreq::continuous
+ * (c++/spectrum.hpp:180-209) with the kernel BR_NEW (c++/spectral.hpp:46-71, log-Gaussian of
+ * width alpha blended into a Gaussian of width omega0 below |E| = omega0).
+ * Inputs (device arrays): the peaks of the positive half (energies pos_e > 0, weights pos_w) and of
+ * the negative half (|energies| neg_e > 0, weights neg_w), as left by Bins::trim; the mesh energies
+ * mesh[k] > 0 (make_mesh, c++/spectrum.hpp:173-178). Outputs: out_pos[k] = A(+mesh[k]) and
+ * out_neg[k] = A(-mesh[k]) (overwritten).
+ */
+int nrg_broaden(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
+                const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
+                double *out_pos, double *out_neg, void *stream);
+
 /* Timing of the most recent nrg_syev_batched / nrg_gemm3_grouped / nrg_fdm_bin call on
  * this context, in milliseconds, measured with HIP events on the call's stream.
  * phase: 0 = whole call, 1.. = internal phases (see nrg_phase_name). */

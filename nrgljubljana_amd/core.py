@@ -70,6 +70,9 @@ def lib():
     L.nrg_fdm_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
                               ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, vp]
     L.nrg_fdm_bin.restype = ctypes.c_int
+    L.nrg_broaden.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_double,
+                              ctypes.c_double, vp, vp, vp]
+    L.nrg_broaden.restype = ctypes.c_int
     L.nrg_last_time_ms.argtypes = [vp, ctypes.c_int]
     L.nrg_last_time_ms.restype = ctypes.c_double
     L.nrg_phase_name.argtypes = [ctypes.c_int]
@@ -156,6 +159,15 @@ class Context:
         if r != 0:
             raise NrgError(r, "nrg_fdm_bin")
 
+    def broaden(self, pos_e, pos_w, neg_e, neg_w, mesh, alpha, omega0, out_pos, out_neg, stream=None):
+        """nrg_broaden on device tensors (float64)."""
+        r = lib().nrg_broaden(self._h, _ptr(pos_e), _ptr(pos_w), ctypes.c_int64(pos_e.numel()), _ptr(neg_e),
+                              _ptr(neg_w), ctypes.c_int64(neg_e.numel()), _ptr(mesh), ctypes.c_int64(mesh.numel()),
+                              ctypes.c_double(alpha), ctypes.c_double(omega0), _ptr(out_pos), _ptr(out_neg),
+                              _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_broaden")
+
     def kernel_timing(self, enable=True):
         lib().nrg_kernel_timing(self._h, int(bool(enable)))
 
@@ -173,6 +185,10 @@ class Context:
 
     def last_time_ms(self, phase=0):
         return lib().nrg_last_time_ms(self._h, int(phase))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr() if t.numel() else 0)
 
 
 def phase_name(phase):

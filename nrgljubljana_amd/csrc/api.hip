@@ -217,6 +217,23 @@ int nrg_fdm_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_b
   }
 }
 
+int nrg_broaden(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
+                const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
+                double *out_pos, double *out_neg, void *stream) {
+  if (!ctx) {
+    set_error("nrg_broaden: NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return broaden_run(ctx, pos_e, pos_w, npos, neg_e, neg_w, nneg, mesh, nmesh, alpha, omega0, out_pos, out_neg,
+                       (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("nrg_broaden: ") + e.what());
+    return NRG_E_HIP;
+  }
+}
+
 double nrg_last_time_ms(nrg_ctx *ctx, int phase) {
   if (!ctx) return -1.0;
   if (phase == 0) return ctx->total_ms;

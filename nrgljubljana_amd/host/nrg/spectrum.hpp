@@ -4,7 +4,8 @@
 // (c++/algo.hpp:55-67) with Algo_FDM (c++/algo_FDM.hpp:123-151) and its two ChainBinning accumulators
 // (greater and lesser halves, c++/spectrum.hpp:12-31): every subspace pair of a step with both operator
 // blocks present is handed to one nrg_fdm_bin call; the step's bins are merged into the running
-// spectrum (SpectrumRealFreq::mergeCFS, c++/spectrum.hpp:73-76) on the host.
+// spectrum (SpectrumRealFreq::mergeCFS, c++/spectrum.hpp:73-76) on the host; continuous() trims the
+// bins (Bins::trim) and broadens them onto the mesh on the GPU (nrg_broaden).
 #pragma once
 #include "dmnrg.hpp"
 
@@ -43,6 +44,66 @@ struct SpectrumBins {
     return s;
   }
 };
+
+// Bins::trim (c++/bins.hpp:164-185): keep bin i if |w_i| >= discard_trim (e_{i+1} - e_i); always
+// keep the last bin. Returns the kept (energy, weight) peaks.
+inline std::pair<std::vector<double>, std::vector<double>> trim(const BinGrid &g, const std::vector<double> &w,
+                                                                double discard_trim) {
+  std::vector<double> e, ww;
+  if (w.empty()) return {e, ww};
+  for (size_t i = 0; i + 1 < w.size(); i++) {
+    const double ei = g.energy(i), width = g.energy(i + 1) - ei;
+    if (std::abs(w[i]) >= discard_trim * width) {
+      e.push_back(ei);
+      ww.push_back(w[i]);
+    }
+  }
+  e.push_back(g.energy(w.size() - 1));
+  ww.push_back(w.back());
+  return {e, ww};
+}
+
+// make_mesh (c++/spectrum.hpp:173-178): E = broaden_max, divided by broaden_ratio while E > broaden_min
+inline std::vector<double> make_mesh(double broaden_max, double broaden_min, double broaden_ratio) {
+  std::vector<double> m;
+  for (double E = broaden_max; E > broaden_min; E /= broaden_ratio) m.push_back(E);
+  return m;
+}
+
+// A broadened spectrum: (energy, value) pairs, negative energies then positive, each ascending, as
+// SpectrumRealFreq::continuous writes them.
+using Curve = std::vector<std::pair<double, double>>;
+
+// SpectrumRealFreq::continuous (c++/spectrum.hpp:180-209) on the GPU (nrg_broaden).
+inline Curve continuous(Device &dev, const BinGrid &g, const SpectrumBins &s, double discard_trim, double alpha,
+                        double omega0, const std::vector<double> &mesh) {
+  const auto [pe, pw] = trim(g, s.pos, discard_trim);
+  const auto [ne, nw] = trim(g, s.neg, discard_trim);
+  auto up = [](const std::vector<double> &v) {
+    DBuf<double> d(v.size() + 1);
+    if (!v.empty()) hip_check(hipMemcpy(d.get(), v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    return d;
+  };
+  DBuf<double> dpe = up(pe), dpw = up(pw), dne = up(ne), dnw = up(nw), dm = up(mesh);
+  DBuf<double> op(mesh.size() + 1), on(mesh.size() + 1);
+  nrg_check(nrg_broaden(dev.ctx(), dpe.get(), dpw.get(), (int64_t)pe.size(), dne.get(), dnw.get(), (int64_t)ne.size(),
+                        dm.get(), (int64_t)mesh.size(), alpha, omega0, op.get(), on.get(), dev.stream()),
+            "nrg_broaden");
+  std::vector<double> vp(mesh.size()), vn(mesh.size());
+  if (!mesh.empty()) {
+    hip_check(hipMemcpy(vp.data(), op.get(), sizeof(double) * mesh.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    hip_check(hipMemcpy(vn.data(), on.get(), sizeof(double) * mesh.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+  }
+  Curve neg, pos;
+  for (size_t k = 0; k < mesh.size(); k++) {
+    neg.emplace_back(-mesh[k], vn[k]);
+    pos.emplace_back(mesh[k], vp[k]);
+  }
+  std::sort(neg.begin(), neg.end());
+  std::sort(pos.begin(), pos.end());
+  neg.insert(neg.end(), pos.begin(), pos.end());
+  return neg;
+}
 
 using FactorFnc = std::function<double(const Invar &, const Invar &)>;   // ff(Ii, Ij)
 using CheckFnc = std::function<bool(const Invar &, const Invar &, int)>;  // cf(Ij, Ii, spin)

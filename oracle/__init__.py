@@ -58,6 +58,13 @@ def lib():
         L.oracle_fdm_bin.restype = ctypes.c_int
         L.oracle_jacobi_ld.argtypes = [ctypes.c_int, dp, dp, dp]
         L.oracle_jacobi_ld.restype = ctypes.c_int
+        L.oracle_broaden.restype = ctypes.c_int
+        L.oracle_br_new.restype = ctypes.c_double
+        L.oracle_br_new.argtypes = [ctypes.c_double] * 4
+        L.oracle_br_l.restype = ctypes.c_double
+        L.oracle_br_l.argtypes = [ctypes.c_double] * 3
+        L.oracle_br_g.restype = ctypes.c_double
+        L.oracle_br_g.argtypes = [ctypes.c_double] * 3
         r = L.oracle_init(blas_path().encode())
         if r != 0:
             raise RuntimeError(f"oracle_init failed ({r})")
@@ -122,3 +129,25 @@ def fdm_bin(pairs, grid, T, wnf, sign, nbins):
     lib().oracle_fdm_bin(arr, len(pairs), ctypes.byref(grid), float(T), float(wnf), int(sign),
                          _dp(out[0]), _dp(out[1]), _dp(out[2]), _dp(out[3]))
     return out
+
+
+def broaden(pos_e, pos_w, neg_e, neg_w, mesh, alpha, omega0):
+    """SpectrumRealFreq::continuous with BR_NEW: returns (A(+mesh), A(-mesh))."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (pos_e, pos_w, neg_e, neg_w, mesh)]
+    out_p, out_n = np.zeros(len(arrs[4])), np.zeros(len(arrs[4]))
+    lib().oracle_broaden(_dp(arrs[0]), _dp(arrs[1]), ctypes.c_int64(len(arrs[0])), _dp(arrs[2]), _dp(arrs[3]),
+                         ctypes.c_int64(len(arrs[2])), _dp(arrs[4]), ctypes.c_int64(len(arrs[4])),
+                         ctypes.c_double(alpha), ctypes.c_double(omega0), _dp(out_p), _dp(out_n))
+    return out_p, out_n
+
+
+def br_new(e, ept, alpha, omega0):
+    return lib().oracle_br_new(e, ept, alpha, omega0)
+
+
+def br_l(e, ept, alpha):
+    return lib().oracle_br_l(e, ept, alpha)
+
+
+def br_g(e, ept, omega0):
+    return lib().oracle_br_g(e, ept, omega0)

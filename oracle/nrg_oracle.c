@@ -17,6 +17,7 @@
  *                         (tasks sorted by decreasing size)
  *   oracle_gemm3          c++/numerics_ublas.hpp:108-140 (product / transform / rotate,
  *                         same evaluation order as the reference)
+ *   oracle_broaden        c++/spectrum.hpp:180-209, c++/spectral.hpp:39-71 (BR_NEW broadening)
  *   oracle_fdm_bin        c++/algo_FDM.hpp:30-60,84-114 (term loops, gt then ls as in
  *                         Algo_FDM::calc 136-142), c++/numerics.hpp:283-306 (prod_fit),
  *                         c++/spectrum.hpp:19-28 (ChainBinning::add),
@@ -330,4 +331,48 @@ int oracle_jacobi_ld(int n, const double *Hup, double *w, double *V) {
   free(a);
   free(v);
   return sweep;
+}
+
+/* ---------------------------------------------------------------------------
+ * Broadening of a binned spectrum (SpectrumRealFreq::continuous, c++/spectrum.hpp:180-209) with
+ * BR_L / BR_G / BR_NEW (c++/spectral.hpp:39-71), summed over the peaks in their order.
+ * ------------------------------------------------------------------------- */
+static const double ORACLE_SQRTPI = 1.7724538509055160273;
+
+double oracle_br_l(double e, double ept, double alpha) {
+  if ((e < 0.0 && ept > 0.0) || (e > 0.0 && ept < 0.0)) return 0.0;
+  if (ept == 0.0) return 0.0;
+  const double gamma = alpha / 4.0;
+  return exp(-pow(log(e / ept) / alpha - gamma, 2)) / (alpha * fabs(e) * ORACLE_SQRTPI);
+}
+
+double oracle_br_g(double e, double ept, double omega0) {
+  return exp(-pow((e - ept) / omega0, 2)) / (omega0 * ORACLE_SQRTPI);
+}
+
+double oracle_br_new(double e, double ept, double alpha, double omega0) {
+  const double part_l = oracle_br_l(e, ept, alpha);
+  if (fabs(e) > omega0) return part_l;
+  const double br_h = exp(-pow(log(fabs(e) / omega0) / alpha, 2));
+  return part_l * br_h + oracle_br_g(e, ept, omega0) * (1.0 - br_h);
+}
+
+int oracle_broaden(const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e, const double *neg_w,
+                   int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0, double *out_pos,
+                   double *out_neg) {
+  for (int64_t k = 0; k < nmesh; k++) {
+    const double E = mesh[k];
+    double vp = 0.0, vn = 0.0;
+    for (int64_t i = 0; i < npos; i++) {
+      vp += pos_w[i] * oracle_br_new(E, pos_e[i], alpha, omega0);
+      vn += pos_w[i] * oracle_br_new(-E, pos_e[i], alpha, omega0);
+    }
+    for (int64_t i = 0; i < nneg; i++) {
+      vn += neg_w[i] * oracle_br_new(-E, -neg_e[i], alpha, omega0);
+      vp += neg_w[i] * oracle_br_new(E, -neg_e[i], alpha, omega0);
+    }
+    out_pos[k] = vp;
+    out_neg[k] = vn;
+  }
+  return 0;
 }

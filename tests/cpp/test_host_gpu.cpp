@@ -16,6 +16,9 @@ int oracle_syev_batched(int nblk, const int64_t *n, const int64_t *off_H, double
 int oracle_gemm3(const nrg_gemm3_term *terms, int nterms);
 int oracle_fdm_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf, int sign,
                    double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg);
+int oracle_broaden(const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e, const double *neg_w,
+                   int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0, double *out_pos,
+                   double *out_neg);
 }
 
 using namespace NRG;
@@ -452,6 +455,49 @@ TEST(Spectrum, fdm_step_matches_oracle) {
     EXPECT_TRUE(err <= 1e-13 * scale);
     std::printf("    step %zu: %zu pairs, max bin deviation %.2e of total |weight| %.3e\n", step, pairs.size(), err, scale);
   }
+}
+
+TEST(Spectrum, continuous_matches_oracle) {
+  // the FDM spectrum of the last step of the toy chain, trimmed and broadened as the reference saves it
+  auto &c = chain();
+  Device &dev = Device::default_device();
+  const size_t N = c.Nmax - 1;
+  DiagInfo &diag = c.diags[N];
+  MatrixElements op;
+  for (const auto &[I, e] : diag) {
+    const Invar Ij{I.qn[0] + 1};
+    if (diag.count(Ij)) op[{Ij, I}] = DMatrix(random_matrix(diag.at(Ij).getnrall(), e.getnrall()));
+  }
+  const BinGrid grid = BinGrid::standard(-1, -1, 1.0, std::pow(2.0, -(double)N / 2), 200);
+  FDMSpectrum spec(dev, grid, -1, c.T);
+  DensMatElements rho = init_rho_FDM(N, c.store, c.stats, c.sym.multfnc(), c.T);
+  spec.calc(diag, op, op, rho, c.stats.wnfactor[N], [](const Invar &, const Invar &) { return 1.0; });
+  const auto mesh = make_mesh(10.0, 3.0 * std::pow(2.0, -(double)N / 2) * 1e-2, 1.05);
+  const SpectrumBins tot = spec.total();
+  const Curve cv = continuous(dev, grid, tot, 1e-16, 0.3, c.T, mesh);
+  EXPECT_EQ(cv.size(), 2 * mesh.size());
+  const auto [pe, pw] = trim(grid, tot.pos, 1e-16);
+  const auto [ne, nw] = trim(grid, tot.neg, 1e-16);
+  std::vector<double> rp(mesh.size()), rn(mesh.size()), sp(mesh.size()), sn(mesh.size());
+  oracle_broaden(pe.data(), pw.data(), (int64_t)pe.size(), ne.data(), nw.data(), (int64_t)ne.size(), mesh.data(),
+                 (int64_t)mesh.size(), 0.3, c.T, rp.data(), rn.data());
+  std::vector<double> apw(pw), anw(nw);
+  for (auto &x : apw) x = std::abs(x);
+  for (auto &x : anw) x = std::abs(x);
+  oracle_broaden(pe.data(), apw.data(), (int64_t)pe.size(), ne.data(), anw.data(), (int64_t)ne.size(), mesh.data(),
+                 (int64_t)mesh.size(), 0.3, c.T, sp.data(), sn.data());
+  double worst = 0;
+  const size_t M = mesh.size();
+  for (size_t k = 0; k < M; k++) {
+    // cv: negative energies ascending (mesh reversed) then positive ascending
+    const double gneg = cv[M - 1 - k].second, gpos = cv[M + (M - 1 - k)].second;
+    EXPECT_NEAR(cv[M + (M - 1 - k)].first, mesh[k], 0.0);
+    worst = std::max(worst, std::abs(gpos - rp[k]) / (sp[k] + 1e-300));
+    worst = std::max(worst, std::abs(gneg - rn[k]) / (sn[k] + 1e-300));
+  }
+  EXPECT_TRUE(worst <= 1e-13);
+  std::printf("    continuous: %zu + %zu peaks after trim, %zu mesh points, max relative deviation %.2e\n", pe.size(),
+              ne.size(), M, worst);
 }
 
 int main(int argc, char **argv) {
