@@ -442,12 +442,74 @@ def bench_fdm(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_broaden(args):
+    """Broadening of one binned spectrum (1000 bins/decade over 14 decades, trimmed) onto the standard
+    460-point mesh (nrg_broaden); value = BR_NEW evaluations per second."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context
+    rng = np.random.default_rng(20261207)
+    n = 14 * 1000 + 1
+    e = 10.0 ** (-12.0 + np.arange(n) / 1000.0)
+    halves = []
+    for _ in range(2):
+        w = np.exp(-((np.log10(e) + 3.0) / 2.0) ** 2) * e * rng.uniform(0.5, 1.5, n)
+        w[rng.uniform(size=n) < 0.3] = 0.0
+        keep = np.append(np.abs(w[:-1]) >= 1e-16 * np.diff(e), True)
+        halves.append((e[keep], w[keep]))
+    mesh, E = [], 10.0
+    while E > 1e-8:
+        mesh.append(E)
+        E /= 1.05
+    mesh = np.array(mesh)
+    dev = torch.device("cuda:0")
+    t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (*halves[0], *halves[1], mesh)]
+    op = torch.zeros(len(mesh), dtype=torch.float64, device=dev)
+    on = torch.zeros_like(op)
+    ctx = Context(0)
+    alpha, omega0 = 0.3, 1e-9
+    evals = 2.0 * len(mesh) * (len(halves[0][0]) + len(halves[1][0]))
+
+    def step():
+        ctx.broaden(*t, alpha, omega0, op, on)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    tms, cnt = ctx.kernel_times()["k_broaden"]
+    ctx.kernel_timing(False)
+    # about 30 FP64 operations per evaluation (log, exp, divisions and the blend), on the vector units
+    ach = 30.0 * evals / (tms / cnt * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_broaden", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "note": "transcendental-bound: ~30 FP64 operations per kernel evaluation counted against the 78.6 TF/s "
+                    "FP64 vector peak"}
+    cpu = None
+    if not args.no_cpu:
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 50:
+            oracle.broaden(*halves[0], *halves[1], mesh, alpha, omega0)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * evals / dt / 1e6, 2), "unit": "Mevals/s", "cores": 1, "kind": "port",
+               "sample": f"the whole spectrum {reps} times, scalar double loop, {dt:.1f} s"}
+    out = {"metric": "spectral broadening (BR_NEW), kernel evaluations per second", "value": round(evals / ms / 1e6, 2),
+           "unit": "Gevals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{len(halves[0][0])} + {len(halves[1][0])} trimmed peaks, {len(mesh)} mesh points, "
+                                  "alpha 0.3, omega0 1e-9, seed 20261207", "evaluations_per_step": evals},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm"], default="eig")
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden"], default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--npairs", type=int, default=48)
@@ -458,7 +520,7 @@ def main():
         print("bench.py: this build runs on one GPU only (the eigensolver shards by blocks across GPUs in "
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm}[args.workload](args)
+    {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden}[args.workload](args)
 
 
 if __name__ == "__main__":
