@@ -489,8 +489,9 @@ TEST(Spectrum, continuous_matches_oracle) {
   double worst = 0;
   const size_t M = mesh.size();
   for (size_t k = 0; k < M; k++) {
-    // cv: negative energies ascending (mesh reversed) then positive ascending
-    const double gneg = cv[M - 1 - k].second, gpos = cv[M + (M - 1 - k)].second;
+    // cv: negative energies ascending (-mesh[k] at k, the mesh descends) then positive ascending
+    const double gneg = cv[k].second, gpos = cv[M + (M - 1 - k)].second;
+    EXPECT_NEAR(cv[k].first, -mesh[k], 0.0);
     EXPECT_NEAR(cv[M + (M - 1 - k)].first, mesh[k], 0.0);
     worst = std::max(worst, std::abs(gpos - rp[k]) / (sp[k] + 1e-300));
     worst = std::max(worst, std::abs(gneg - rn[k]) / (sn[k] + 1e-300));
