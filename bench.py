@@ -168,6 +168,21 @@ def bench_eig(args):
     ms_step = 1e3 * elapsed / args.steps
     syev_ms = ctx.last_time_ms(0)
 
+    # the same call with the input coming from, and the eigenpairs going back to, pinned host memory
+    # (what a caller holding host matrices pays; not part of `value`)
+    host_in = torch.from_numpy(arena).pin_memory()
+    host_vec = torch.empty_like(host_in).pin_memory()
+    host_w = torch.empty(w.shape, dtype=torch.float64).pin_memory()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        H.copy_(host_in, non_blocking=True)
+        ctx.syev_batched(H, sizes, off_H, w, off_w, mode)
+        host_vec.copy_(H, non_blocking=True)
+        host_w.copy_(w, non_blocking=True)
+    torch.cuda.synchronize()
+    pcie_ms = 1e3 * (time.perf_counter() - t0) / 2
+
     # live per-kernel timing (HIP events on the library's stream) over two more steps
     ctx.kernel_timing(True)
     for _ in range(2):
@@ -221,6 +236,7 @@ def bench_eig(args):
                    "form": args.mode, "blocks": int(len(sizes)), "sum_n": int(sizes.sum()), "max_n": int(sizes.max()),
                    "flops_per_step": flops, "alg_bytes_per_step": abytes},
         "syev_call_ms": round(syev_ms, 3),
+        "pcie_inclusive_ms_per_step": round(pcie_ms, 3),
         "frac_fp64_peak": round(gflops / 1e3 / FP64_PEAK_TFLOPS, 4),
         "max_eig_err_rel": eig_err,
         "max_orth_err": orth_err,
@@ -504,12 +520,94 @@ def bench_broaden(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_dm(args):
+    """One backward step of the density-matrix sweep: all cdmI rotations of the step as one grouped
+    nrg_gemm3_grouped call (FP64); value = GFLOP/s in the reference's operation order."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, NRG_GEMM_FP64, Gemm3Term, workloads
+    step = workloads.dm_step(nsub=args.npairs)
+    flops = workloads.dm_flops(step)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(20261208)
+    keep, terms, outs, host = [], [], [], []
+    for sub in step:
+        C = torch.zeros(sub["kept"], sub["kept"], dtype=torch.float64, device=dev)
+        outs.append(C)
+        for ch in sub["children"]:
+            rho = rng.standard_normal((ch["nromega"], ch["nromega"]))
+            U = rng.standard_normal((ch["nromega"], ch["total"])) / np.sqrt(ch["total"])
+            dr, dU = torch.from_numpy(rho).to(dev), torch.from_numpy(U).to(dev)
+            keep += [dr, dU]
+            host.append((rho, U))
+            t = Gemm3Term()
+            t.m = t.n = sub["kept"]; t.k = t.l = ch["nromega"]; t.f = ch["f"]
+            t.A = dU.data_ptr() + 8 * ch["off"]; t.lda = ch["total"]; t.transA = 1
+            t.O = dr.data_ptr(); t.ldo = ch["nromega"]
+            t.B = dU.data_ptr() + 8 * ch["off"]; t.ldb = ch["total"]; t.transB = 1
+            t.C = C.data_ptr(); t.ldc = sub["kept"]
+            terms.append(t)
+    ctx = Context(0)
+
+    def call():
+        for C in outs:
+            C.zero_()
+        ctx.gemm3_grouped(terms, NRG_GEMM_FP64)
+    ms = timed_calls(call, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        call()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log(f"dm step: {len(step)} subspaces x 16 children, {flops / 1e9:.2f} GFLOP, {ms:.3f} ms per call; kernels "
+        + ", ".join(f"{k} {v[0] / 2:.3f} ms" for k, v in kt.items()))
+    gemm_ms = sum(v[0] for k, v in kt.items() if k.startswith("k_gemm")) / 2
+    ach = flops / (gemm_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": "k_gemm_f64 + k_gemm_f64_big", "achieved": round(ach, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None}
+    cpu = None
+    if not args.no_cpu:
+        L = oracle.lib()
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        L.oracle_blas_threads(cores)
+        hterms, hold, hi = [], [], 0
+        for sub in step:
+            C = np.zeros((sub["kept"], sub["kept"]))
+            hold.append(C)
+            for ch in sub["children"]:
+                rho, U = host[hi]
+                hi += 1
+                t = Gemm3Term()
+                t.m = t.n = sub["kept"]; t.k = t.l = ch["nromega"]; t.f = ch["f"]
+                t.A = U.ctypes.data + 8 * ch["off"]; t.lda = ch["total"]; t.transA = 1
+                t.O = rho.ctypes.data; t.ldo = ch["nromega"]
+                t.B = U.ctypes.data + 8 * ch["off"]; t.ldb = ch["total"]; t.transB = 1
+                t.C = C.ctypes.data; t.ldc = sub["kept"]
+                hterms.append(t)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 100:
+            oracle.gemm3(hterms)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+               "sample": f"the whole step {reps} times, oracle dgemm pairs in the reference's rotate order, {dt:.1f} s"}
+    out = {"metric": "density-matrix sweep step (cdmI rotations) GFLOP/s", "value": round(flops / ms / 1e6, 2),
+           "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{len(step)} subspaces of step N-1, kept log-uniform [16,1000], 16 children each, "
+                                  "seed 20261207", "flops_per_step": flops},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden"], default="eig")
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm"], default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--npairs", type=int, default=48)
@@ -520,7 +618,8 @@ def main():
         print("bench.py: this build runs on one GPU only (the eigensolver shards by blocks across GPUs in "
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden}[args.workload](args)
+    {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
+     "dm": bench_dm}[args.workload](args)
 
 
 if __name__ == "__main__":

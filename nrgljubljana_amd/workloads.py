@@ -133,3 +133,35 @@ def fdm_step(npairs=96, dmin=16, dmax=2000, keep_frac=0.4, emax=10.0, seed=SEED 
 def fdm_elements(pairs):
     """Elements evaluated per half (gt or ls): |D_i||D_j| + |D_i||K_j| + |K_i||D_j|."""
     return sum(p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"] for p in pairs)
+
+
+def dm_step(nsub=48, combs=16, kmin=16, kmax=1000, seed=SEED + 4):
+    """One backward step of the density-matrix sweep (cdmI, c++/dmnrg.hpp:54-80): nsub subspaces of
+    step N-1 with kept dimensions log-uniform in [kmin, kmax]; each has `combs` children at step N
+    whose ancestor block i has rmax = the parent's kept dimension; child c holds rho_N (nromega x
+    nromega, nromega = its kept states) and eigenvectors U (nromega x its total dimension), of which
+    the columns part(i) enter: rho_{N-1}(I) += f U(:, part)^T rho_N(child) U(:, part)."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(kmin), np.log(kmax), size=nsub)
+    kept = np.clip(np.rint(np.exp(u)), kmin, kmax).astype(np.int64)
+    out = []
+    for k in kept:
+        children = []
+        for c in range(combs):
+            nrom = int(np.clip(np.rint(np.exp(rng.uniform(np.log(kmin), np.log(kmax)))), kmin, kmax))
+            total = int(k) + int(rng.integers(0, 2 * kmax))   # the parent's block plus the other ancestors'
+            off = int(rng.integers(0, total - int(k) + 1))
+            children.append(dict(nromega=nrom, total=total, off=off, f=float(rng.uniform(0.3, 1.0))))
+        out.append(dict(kept=int(k), children=children))
+    return out
+
+
+def dm_flops(step):
+    """rotate in the reference's order (c++/numerics_ublas.hpp:130-140): T = U^T rho (2 rmax nromega^2),
+    then M += f T U (2 rmax^2 nromega), per (subspace, child)."""
+    s = 0.0
+    for sub in step:
+        r = sub["kept"]
+        for ch in sub["children"]:
+            s += 2.0 * r * ch["nromega"] ** 2 + 2.0 * r * r * ch["nromega"]
+    return s
