@@ -49,14 +49,24 @@ inline int gemm_tiles(const GemmDescT<T> &g) {
   return ((g.M + t - 1) / t) * ((g.N + t - 1) / t);
 }
 
+// Fewer big tiles than this in one launch leave most compute units idle: the launch then uses
+// the small tiles only (four per big tile).
+constexpr int GEMM_BIG_MIN_TILES = 160;
+
 // appends (desc index, tile) items of descriptors [d0, d1) to `items`: first every big tile (tile
 // index with GEMM_BIG_FLAG), then every small one, so that gemm_launch can split the list
 template <typename T>
 inline void gemm_items(const std::vector<GemmDescT<T>> &d, size_t d0, size_t d1, std::vector<int> &items) {
+  size_t nbig = 0;
+  for (size_t i = d0; i < d1; i++)
+    if (gemm_is_big(d[i])) nbig += gemm_tiles(d[i]);
+  const bool use_big = nbig >= (size_t)GEMM_BIG_MIN_TILES;
   for (int big = 1; big >= 0; big--)
     for (size_t i = d0; i < d1; i++) {
-      if ((int)gemm_is_big(d[i]) != big) continue;
-      const int t = gemm_tiles(d[i]);
+      const bool b = use_big && gemm_is_big(d[i]);
+      if ((int)b != big) continue;
+      const int ts = b ? GEMM_BIG : GEMM_TILE;
+      const int t = ((d[i].M + ts - 1) / ts) * ((d[i].N + ts - 1) / ts);
       for (int k = 0; k < t; k++) {
         items.push_back((int)i);
         items.push_back(big ? (k | GEMM_BIG_FLAG) : k);

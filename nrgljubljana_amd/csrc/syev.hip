@@ -35,7 +35,7 @@ constexpr int NT_PANEL = 1024; // threads of the column-preparation kernel
 constexpr int SMALLM = 64;    // merges up to this order do their own block-diagonal product
 
 struct BlkDesc {
-  int64_t n, offA, offZ, offV, offWp, offT, offH, offw, offYp;
+  int64_t n, offA, offZ, offV, offWp, offT, offH, offw, offYp, offDt;
   int L, P;
 };
 
@@ -139,10 +139,15 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items, 
 
 // ------------------------------------------------------------------------------------------
 // 2a. column preparation (dlatrd, lower): one workgroup per block and reduced column.
-//     Finishes the W column of the previous reflector from the tile partials of k_col_symv,
-//     applies the panel's earlier reflectors to column jl and generates reflector jl.
+//     Finishes the W column of the previous reflector from the partials of k_col_symv, applies
+//     the panel's earlier reflectors to column jl and generates reflector jl. A thread owns the
+//     rows tid + q*NT_PANEL of the trailing part and issues every load of a row together, so a
+//     column costs a few memory round trips instead of one per dot product.
 // ------------------------------------------------------------------------------------------
-template <typename T>
+constexpr int KCH = 16;   // panel columns loaded per batch
+
+// RPT: rows per thread, 1 when every block of the launch has at most NT_PANEL rows [jl, m)
+template <typename T, int RPT>
 __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int *blist, int p, int jl) {
   const BlkDesc bd = a.bd[blist[blockIdx.x]];
   const int n = (int)bd.n;
@@ -154,131 +159,155 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   double *e = a.e + bd.offV + i0;
   T *tau = a.tau + bd.offV + i0;
   T *vb = a.vb + bd.offV + i0;                   // reflector handed to k_col_symv
-  const T *yp = a.Yp + bd.offYp + i0;            // tile partials: yp[slot*n + r]
+  const T *yp = a.Yp + bd.offYp + i0;            // symmetric-product partials: yp[slot*n + r]
+  const T *dt = a.Yp + bd.offDt;                 // per 64-row segment: W^T v (0..31), V^T v (32..63)
   const bool final = (m - 1 <= NB);
   const int w = final ? m - 1 : NB;
+  const bool do_col = jl < w || final;           // column jl is updated here (and reduced if jl < w)
+  const int jp = jl - 1;                         // reflector whose W column is finished here
   const int tid = threadIdx.x;
   constexpr int NW = NT_PANEL / WAVE;
-  const int lane = tid & (WAVE - 1), wid = tid / WAVE;
+  constexpr int KC = RPT > 1 ? KCH / 2 : KCH;   // fewer columns in flight when a thread owns several rows
 
-  __shared__ T vs[NMAX];
-  __shared__ T vrow[NB], wrow[NB], cw[NB], cv[NB];
+  __shared__ T part[NW][64];
+  __shared__ T cwv[2 * NB];          // cw[k] = W(:,k)^T v and cv[k] = V(:,k)^T v over the trailing rows
+  __shared__ T vrow[NB], wrow[NB];   // A(jl,k) and W(jl,k)
   __shared__ T red[NW];
+  __shared__ T sh[2];                // W(jl,jp); A(jl+1,jl) after the update
 
-  // 1. W(:, jl-1) = tau (A22 v - V (W^T v) - W (V^T v)) - tau/2 (w^T v) v
+  const T tv = jl > 0 ? tau[jp] : (T)0;
+  const int Tt = (m - jl + 63) / 64;             // 64-row segments of the rows [jl, m)
+  T v[RPT], y[RPT], c[RPT], aj[RPT];
+  // 1. loads that depend on nothing computed here
   if (jl > 0) {
-    const int jp = jl - 1;
-    const T tv = tau[jp];
-    T *Wj = W + (size_t)jp * n;
-    if (tv != (T)0) {
-      const int t = m - jp - 1, Tt = (t + 63) / 64;
-      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) vs[r] = vb[r];
-      __syncthreads();
-      if (jp > 0) {
-        for (int k = wid; k < jp; k += NW) {
-          T sw = 0, sv = 0, sw2 = 0, sv2 = 0;
-          const T *Wk = W + (size_t)k * n, *Ak = A + (size_t)k * n;
-          int r = jp + 1 + lane;
-          for (; r + WAVE < m; r += 2 * WAVE) {
-            const T w0 = Wk[r], a0 = Ak[r], w1 = Wk[r + WAVE], a1 = Ak[r + WAVE];
-            sw += w0 * vs[r];
-            sv += a0 * vs[r];
-            sw2 += w1 * vs[r + WAVE];
-            sv2 += a1 * vs[r + WAVE];
-          }
-          if (r < m) {
-            sw += Wk[r] * vs[r];
-            sv += Ak[r] * vs[r];
-          }
-          sw += sw2;
-          sv += sv2;
-          sw = wave_sum(sw);
-          sv = wave_sum(sv);
-          if (lane == 0) {
-            cw[k] = sw;
-            cv[k] = sv;
-          }
-        }
-        __syncthreads();
+    const int cl = tid & 63;
+    T s = 0;
+    if ((cl & 31) < jp)
+      for (int sg = tid >> 6; sg < Tt; sg += NW) s += dt[sg * 64 + cl];
+    part[tid >> 6][cl] = s;
+  }
+  if (tid < NB) {
+    vrow[tid] = (do_col && tid < jl) ? A[(size_t)tid * n + jl] : (T)0;
+    wrow[tid] = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = jl + tid + q * NT_PANEL;
+    v[q] = y[q] = c[q] = aj[q] = 0;
+    if (r < m) {
+      if (jl > 0) {
+        v[q] = vb[r];
+        aj[q] = A[(size_t)jp * n + r];
+        T s = 0;
+#pragma unroll 16
+        for (int sl = 0; sl < Tt; sl++) s += yp[(size_t)sl * n + r];
+        y[q] = s;
       }
-      T dp = 0;
-      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) {
-        T y = 0, y2 = 0;
-#pragma unroll 8
-        for (int sl = 0; sl < Tt; sl++) y += yp[(size_t)sl * n + r];
-#pragma unroll 8
-        for (int k = 0; k < jp; k++) {
-          y2 += A[(size_t)k * n + r] * cw[k];
-          y2 += W[(size_t)k * n + r] * cv[k];
-        }
-        y -= y2;
-        y *= tv;
-        Wj[r] = y;
-        dp += y * vs[r];
-      }
-      const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
-      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) Wj[r] += a2 * vs[r];
-    } else {
-      for (int r = jp + 1 + tid; r < m; r += NT_PANEL) Wj[r] = 0;
+      if (do_col) c[q] = A[(size_t)jl * n + r];
+    }
+  }
+  if (jl > 0) {
+    __syncthreads();
+    if (tid < 2 * NB) {
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < NW; q++) s += part[q][tid];
+      cwv[tid] = s;
     }
     __syncthreads();
-  }
-  if (jl < w) {
-    // 2. apply the panel's earlier reflectors to column jl
-    if (jl > 0) {
-      if (tid < jl) {
-        vrow[tid] = A[(size_t)tid * n + jl];
-        wrow[tid] = W[(size_t)tid * n + jl];
-      }
-      __syncthreads();
-      for (int r = jl + tid; r < m; r += NT_PANEL) {
-        T acc = 0, acc2 = 0;
-#pragma unroll 8
-        for (int k = 0; k < jl; k++) {
-          acc += A[(size_t)k * n + r] * wrow[k];
-          acc2 += W[(size_t)k * n + r] * vrow[k];
+    // 2. W(:,jp) = tau (A22 v - V cw - W cv) - tau/2 (w^T v) v, and the terms k < jp of the update
+    //    of column jl: A(:,jl) -= V W(jl,:)^T + W V(jl,:)^T
+    T dp = 0;
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int r = jl + tid + q * NT_PANEL;
+      if (r < m) {
+        T ay = 0, ac = 0;
+        for (int k0 = 0; k0 < jp; k0 += KC) {
+          T ak[KC], wk[KC];
+#pragma unroll
+          for (int k = 0; k < KC; k++) {
+            const bool in = k0 + k < jp;
+            ak[k] = in ? A[(size_t)(k0 + k) * n + r] : (T)0;
+            wk[k] = in ? W[(size_t)(k0 + k) * n + r] : (T)0;
+          }
+#pragma unroll
+          for (int k = 0; k < KC; k++) {
+            ay += ak[k] * cwv[k0 + k] + wk[k] * cwv[NB + k0 + k];
+            ac += ak[k] * wrow[k0 + k] + wk[k] * vrow[k0 + k];
+          }
         }
-        A[(size_t)jl * n + r] -= acc + acc2;
+        const T yy = tv == (T)0 ? (T)0 : tv * (y[q] - ay);
+        y[q] = yy;
+        dp += yy * v[q];
+        c[q] -= ac;
       }
-      __syncthreads();
     }
-    // 3. reflector annihilating A(jl+2:m, jl) (dlarfg)
-    T *colj = A + (size_t)jl * n;
-    const T alpha = colj[jl + 1];
-    T part = 0;
-    for (int r = jl + 2 + tid; r < m; r += NT_PANEL) part += colj[r] * colj[r];
-    const T xn2 = block_sum<NT_PANEL>(part, red);
-    T beta, tv, scal = 1;
-    if (xn2 == (T)0) {
-      tv = 0;
-      beta = alpha;
-    } else {
-      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-      tv = (beta - alpha) / beta;
-      scal = (T)1 / (alpha - beta);
+    const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int r = jl + tid + q * NT_PANEL;
+      if (r < m) {
+        y[q] += a2 * v[q];
+        W[(size_t)jp * n + r] = y[q];
+        if (r == jl) sh[0] = y[q];
+      }
     }
-    for (int r = jl + 1 + tid; r < m; r += NT_PANEL) {
-      if (r == jl + 1) {
+    __syncthreads();
+    if (do_col) {
+      const T wj = sh[0], vj = vrow[jp];
+#pragma unroll
+      for (int q = 0; q < RPT; q++) c[q] -= aj[q] * wj + y[q] * vj;
+    }
+  }
+  if (!do_col) return;
+  if (jl == w) {   // last diagonal element of the block (row jl is owned by thread 0)
+    if (tid == 0) {
+      A[(size_t)jl * n + jl] = c[0];
+      d[jl] = (double)c[0];
+    }
+    return;
+  }
+  // 3. reflector annihilating A(jl+2:m, jl) (dlarfg)
+  T nrm = 0;
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = jl + tid + q * NT_PANEL;
+    if (r >= jl + 2 && r < m) nrm += c[q] * c[q];
+    if (r == jl + 1) sh[1] = c[q];
+  }
+  const T xn2 = block_sum<NT_PANEL>(nrm, red);
+  const T alpha = sh[1];
+  T beta, tn, scal = 1;
+  if (xn2 == (T)0) {
+    tn = 0;
+    beta = alpha;
+  } else {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tn = (beta - alpha) / beta;
+    scal = (T)1 / (alpha - beta);
+  }
+  T *colj = A + (size_t)jl * n;
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = jl + tid + q * NT_PANEL;
+    if (r < m) {
+      if (r == jl) {
+        colj[r] = c[q];
+        d[jl] = (double)c[q];
+      } else if (r == jl + 1) {
+        colj[r] = 1;
         vb[r] = 1;
       } else {
-        const T x = colj[r] * scal;
+        const T x = c[q] * scal;
         colj[r] = x;
         vb[r] = x;
       }
     }
-    __syncthreads();
-    if (tid == 0) {
-      e[jl] = (double)beta;
-      tau[jl] = tv;
-      colj[jl + 1] = 1;
-      d[jl] = (double)colj[jl];
-    }
-  } else if (final && tid == 0) {
-    // last diagonal element of the block: apply the panel's reflectors
-    T dd = A[(size_t)(m - 1) * n + (m - 1)];
-    for (int k = 0; k < w; k++) dd -= (T)2 * A[(size_t)k * n + (m - 1)] * W[(size_t)k * n + (m - 1)];
-    A[(size_t)(m - 1) * n + (m - 1)] = dd;
-    d[m - 1] = (double)dd;
+  }
+  if (tid == 0) {
+    e[jl] = (double)beta;
+    tau[jl] = tn;
   }
 }
 
@@ -344,21 +373,42 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
   }
   const BlkDesc bd = a.bd[blist[lo]];
   const int tl = g - tpre[lo];
-  int I = (int)((sqrt(8.0 * tl + 1.0) - 1.0) * 0.5);
-  while ((I + 1) * (I + 2) / 2 <= tl) I++;
-  while (I * (I + 1) / 2 > tl) I--;
-  const int J = tl - I * (I + 1) / 2;
   const int n = (int)bd.n;
   const int i0 = p * NB;
   const int m = n - i0;
   const int r0 = jl + 1;                          // trailing rows/cols [r0, m) of the panel
   const int t = m - r0;
+  const int nseg = (t + 63) / 64, ntri = nseg * (nseg + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tl >= ntri) {
+    // segment partials of W^T v and V^T v for the next column's k_col_prep (only columns k < jl
+    // exist): wave 0/1 take W(:, 0..15 / 16..31), wave 2/3 take V(:, 0..15 / 16..31)
+    const int I = tl - ntri;
+    const int kb = (wv & 1) * KCH;
+    if (kb >= jl) return;
+    const int r = I * 64 + lane;
+    const bool rin = r < t;
+    const T *P = wv < 2 ? a.Wp + bd.offWp + r0 : a.A + bd.offA + (size_t)i0 * n + i0 + r0;
+    const T vr = rin ? a.vb[bd.offV + i0 + r0 + r] : (T)0;
+    T z[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) z[q] = (rin && kb + q < jl) ? P[(size_t)(kb + q) * n + r] : (T)0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) z[q] *= vr;
+    const T s = butterfly16(z);
+    const int cl = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+    if ((lane & 3) == 0) a.Yp[bd.offDt + I * 64 + (wv >> 1) * 32 + kb + cl] = s;
+    return;
+  }
+  int I = (int)((sqrt(8.0 * tl + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= tl) I++;
+  while (I * (I + 1) / 2 > tl) I--;
+  const int J = tl - I * (I + 1) / 2;
   const T *S = a.A + bd.offA + (size_t)(i0 + r0) * n + (i0 + r0);   // S(r,c) = S[c*n + r]
   const T *v = a.vb + bd.offV + i0 + r0;
   T *yp = a.Yp + bd.offYp + i0 + r0;             // yp[slot*n + r]
   __shared__ T vI[64], vJ[64];
   __shared__ T red[4][64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid < 64) {
     const int r = I * 64 + tid;
     vI[tid] = r < t ? v[r] : (T)0;
@@ -973,9 +1023,13 @@ __global__ __launch_bounds__(256) void k_bt_ymat(SyevArgs<T> a, const int2 *item
       Y[(size_t)j * n + r] = r == j + 1 ? (T)1 : (r > j + 1 ? A[(size_t)j * n + r] : (T)0);
 }
 
-// T_s (kb x kb, upper triangular, column-major) from G_s and the reflector scalars
+// T_s (kb x kb, upper triangular, column-major) of super-panel s from G_s = Y_s^T Y_s and the
+// reflector scalars (dlarft, forward, columnwise), blocked: the four diagonal 32x32 blocks follow
+// the column recurrence T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), one wave each; the blocks above
+// them follow from T12 = -T11 G12 T22 (the product of two compact-WY transforms).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<T> a, const int2 *items, const int64_t *toff) {
+  constexpr int BS = 32, NBS = KBT / BS;
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
   const int n = (int)bd.n, sp = it.y;
@@ -984,22 +1038,67 @@ __global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<T> a, const int2 *item
   const T *G = a.Gb + off;     // G(c, i) = G[i*kb + c]
   T *Tg = a.Tb + off;
   const T *tau = a.tau + bd.offV + sp * KBT;
-  __shared__ T Ts[KBT][KBT + 1];
-  __shared__ T gcol[KBT];
-  for (int i = 0; i < kb; i++) {
-    const T ti = tau[i];
-    for (int c = threadIdx.x; c < i; c += 256) gcol[c] = G[(size_t)i * kb + c];
-    __syncthreads();
-    for (int x = threadIdx.x; x < i; x += 256) {
-      T sum = 0;
-      for (int c = x; c < i; c++) sum += Ts[x][c] * gcol[c];
-      Ts[x][i] = -ti * sum;
+  __shared__ T Tp[KBT * (KBT + 1) / 2];   // upper triangle, T(x, i) at i(i+1)/2 + x
+  __shared__ T Gs[NBS * BS * (BS + 1)];   // diagonal blocks of G; then G(0:base, block)
+  __shared__ T Ys[(KBT - BS) * BS];       // T(0:base, 0:base) G(0:base, block)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  auto P = [](int x, int i) { return i * (i + 1) / 2 + x; };
+  for (int idx = tid; idx < NBS * BS * BS; idx += 256) {
+    const int b = idx / (BS * BS), c = idx % BS, i = (idx / BS) % BS;
+    const int gc = b * BS + c, gi = b * BS + i;
+    Gs[(b * BS + c) * (BS + 1) + i] = (gc < kb && gi < kb) ? G[(size_t)gi * kb + gc] : (T)0;
+  }
+  __syncthreads();
+  // 1. diagonal blocks
+  {
+    const int base = wv * BS, x = lane;
+    const T *Gb = Gs + base * (BS + 1);   // G(base + c, base + i) at Gb[c*(BS+1) + i]
+    for (int i = 0; i < BS; i++) {
+      const int gi = base + i;
+      if (gi < kb && x <= i) {
+        const T ti = tau[gi];
+        if (x < i) {
+          T s = 0;
+          for (int c = x; c < i; c++) s += Tp[P(base + x, base + c)] * Gb[c * (BS + 1) + i];
+          Tp[P(base + x, gi)] = -ti * s;
+        } else {
+          Tp[P(gi, gi)] = ti;
+        }
+      }
+      __syncthreads();
     }
-    if (threadIdx.x == 0) Ts[i][i] = ti;
-    for (int x = i + 1 + threadIdx.x; x < kb; x += 256) Ts[x][i] = (T)0;
+  }
+  // 2. blocks above the diagonal, one column block at a time
+  for (int b = 1; b < NBS; b++) {
+    const int base = b * BS;
+    if (base >= kb) break;
+    const int jw = min(BS, kb - base);
+    for (int idx = tid; idx < base * BS; idx += 256) {
+      const int c = idx / BS, j = idx % BS;
+      Gs[idx] = j < jw ? G[(size_t)(base + j) * kb + c] : (T)0;   // X(c, j) = G(c, base + j)
+    }
+    __syncthreads();
+    for (int idx = tid; idx < base * BS; idx += 256) {
+      const int x = idx / BS, j = idx % BS;
+      T s = 0;
+      for (int c = x; c < base; c++) s += Tp[P(x, c)] * Gs[c * BS + j];
+      Ys[idx] = s;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < base * BS; idx += 256) {
+      const int x = idx / BS, j = idx % BS;
+      if (j < jw) {
+        T s = 0;
+        for (int c = 0; c <= j; c++) s += Ys[x * BS + c] * Tp[P(base + c, base + j)];
+        Tp[P(x, base + j)] = -s;
+      }
+    }
     __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < kb * kb; idx += 256) Tg[idx] = Ts[idx % kb][idx / kb];
+  for (int idx = tid; idx < kb * kb; idx += 256) {
+    const int x = idx % kb, i = idx / kb;
+    Tg[idx] = x <= i ? Tp[P(x, i)] : (T)0;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1495,7 +1594,7 @@ struct SyevPlan {
   // work lists, all packed into one int array; offsets in ints
   std::vector<int> ibuf;
   size_t symm_off = 0, symm_cnt = 0;
-  struct ColLaunch { int p, jl; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };
+  struct ColLaunch { int p, jl, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };
   std::vector<ColLaunch> cols;
   std::vector<size_t> upd_off, upd_cnt;
   int64_t sumYp = 0;
@@ -1538,6 +1637,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     d.P = panels_of(n[b]);
     d.offT = to; to += (int64_t)d.P * NB * NB;
     d.offYp = pl.sumYp; pl.sumYp += n[b] * ((n[b] + 63) / 64);
+    d.offDt = pl.sumYp; pl.sumYp += 64 * ((n[b] + 63) / 64);
     d.offH = off_H[b];
     d.offw = off_w[b];
     d.L = 31 - __builtin_clz((unsigned)n[b]);
@@ -1576,7 +1676,12 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
       SyevPlan::ColLaunch c;
       c.p = p; c.jl = jl;
       c.prep_off = I.size();
-      for (int b : order) if (pl.bd[b].P > p && jl <= width(b)) I.push_back(b);
+      c.rows = 0;
+      for (int b : order)
+        if (pl.bd[b].P > p && jl <= width(b)) {
+          I.push_back(b);
+          c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * p - jl));
+        }
       c.prep_cnt = I.size() - c.prep_off;
       c.symv_off = I.size();
       for (int b : order) if (pl.bd[b].P > p && jl < width(b)) I.push_back(b);
@@ -1588,7 +1693,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
         const int64_t t = n[b] - (int64_t)NB * p - jl - 1;
         const int64_t T = (t + 63) / 64;
         I.push_back((int)tiles);
-        tiles += T * (T + 1) / 2;
+        tiles += T * (T + 1) / 2 + (jl > 0 ? T : 0);   // + one dot-product workgroup per segment
       }
       c.tiles = (size_t)tiles;
       pl.cols.push_back(c);
@@ -1740,8 +1845,18 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
       g.M = mt; g.N = mt; g.alpha = -1.0; g.beta = 1.0;
       gd.push_back(g);
     }
+    // only the tiles on or below the first tile super-diagonal: the reduction reads the lower
+    // triangle, and the symmetric product's diagonal 64x64 tiles reach at most 63 columns above it
     pl.s2_items[p] = I.size();
-    gemm_items(gd, d0, gd.size(), I);
+    for (size_t i = d0; i < gd.size(); i++) {
+      const int tm = (gd[i].M + GEMM_TILE - 1) / GEMM_TILE;
+      const int tiles = tm * ((gd[i].N + GEMM_TILE - 1) / GEMM_TILE);
+      for (int k = 0; k < tiles; k++)
+        if (k / tm <= k % tm + 1) {
+          I.push_back((int)i);
+          I.push_back(k);
+        }
+    }
     pl.s2_cnt[p] = (I.size() - pl.s2_items[p]) / 2;
   }
   // back-transformation products
@@ -1852,9 +1967,14 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   for (int p = 0; p < pl.Pmax; p++) {
     for (; ci < pl.cols.size() && pl.cols[ci].p == p; ci++) {
       const SyevPlan::ColLaunch &c = pl.cols[ci];
-      if (c.prep_cnt)
-        NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL(k_col_prep<T>, grid(c.prep_cnt), dim3(NT_PANEL), 0, st, aT,
-                                                            I + c.prep_off, p, c.jl));
+      if (c.prep_cnt) {
+        if (c.rows <= NT_PANEL)
+          NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, st, aT,
+                                                              I + c.prep_off, p, c.jl));
+        else
+          NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, NMAX / NT_PANEL>), grid(c.prep_cnt), dim3(NT_PANEL),
+                                                              0, st, aT, I + c.prep_off, p, c.jl));
+      }
       if (c.tiles)
         NRG_TIMED(ctx, st, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, st, aT,
                                                             I + c.symv_off, I + c.tpre_off, (int)c.nact, p, c.jl));
