@@ -9,7 +9,7 @@
 //      k_col_symv   algorithm, lower variant): per reduced column, one workgroup per block prepares
 //                   the reflector and many workgroups per block form the symmetric product over
 //                   64x64 tiles of the lower triangle;
-//      k_syr2k      after each panel of NB columns, the trailing rank-2k update A22 -= V W^T + W V^T
+//      k_gemm_f64   after each panel of NB columns, the trailing rank-2k update A22 -= V W^T + W V^T
 //   3. k_dc_prep    divide and conquer on the tridiagonal matrix (Cuppen's method with the
 //      k_dc_*       Gu-Eisenstat eigenvector formula, as LAPACK dstedc/dlaed0-3): leaves of order
 //      k_gemm_f64   1-2 at a common tree depth, then five merge launches per depth for all blocks
@@ -32,11 +32,20 @@ namespace nrg {
 
 constexpr int NB = 32;        // panel width of the Householder reduction
 constexpr int NT_PANEL = 1024; // threads of the column-preparation kernel
+constexpr int TAIL = 128;      // trailing order reduced inside one workgroup's LDS (k_tail)
 constexpr int SMALLM = 64;    // merges up to this order do their own block-diagonal product
+// work-list entry of the column kernels: block index in the low bits, its panel above them
+constexpr int COL_PANEL_SHIFT = 20;
+constexpr int COL_BLK_MASK = (1 << COL_PANEL_SHIFT) - 1;
+inline int col_entry(int b, int p) { return b | (p << COL_PANEL_SHIFT); }
+// Panel of a block with P panels in panel slot q (-1: none). Every block starts in slot 0; a
+// right-aligned schedule (all blocks ending together) measured no faster, since the bandwidth of
+// the first columns belongs to the largest blocks either way.
+inline int panel_in_slot(int q, int P) { return q < P ? q : -1; }
 
 struct BlkDesc {
-  int64_t n, offA, offZ, offV, offWp, offT, offH, offw, offYp, offDt;
-  int L, P;
+  int64_t n, offA, offZ, offV, offWp, offH, offw, offYp, offDt;
+  int L, P;   // P: panels reduced by the column kernels (before the tail)
 };
 
 template <typename T>
@@ -45,7 +54,6 @@ struct SyevArgs {
   T *A;            // work matrices (column-major, full symmetric, then reflectors)
   T *Wp;           // dlatrd W panels, n x NB per block
   T *tau;          // Householder scalars, n per block
-  T *Tf;           // compact-WY T factors, P x NB x NB per block
   T *vb;           // current reflector of every block, n per block
   T *Yp;           // symmetric-product tile partials, n x ceil(n/64) per block
   T *Ym, *Yh;      // back-transformation: explicit reflectors Y and Yhat = Y T^T (n x n per block)
@@ -148,8 +156,9 @@ constexpr int KCH = 16;   // panel columns loaded per batch
 
 // RPT: rows per thread, 1 when every block of the launch has at most NT_PANEL rows [jl, m)
 template <typename T, int RPT>
-__global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int *blist, int p, int jl) {
-  const BlkDesc bd = a.bd[blist[blockIdx.x]];
+__global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int *blist, int jl) {
+  const int ent = blist[blockIdx.x], p = ent >> COL_PANEL_SHIFT;
+  const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
   const int n = (int)bd.n;
   const int i0 = p * NB;
   const int m = n - i0;
@@ -362,8 +371,7 @@ __device__ __forceinline__ T butterfly16(T (&x)[16]) {
 //     segment is written exactly once, so the sum k_col_prep forms is deterministic.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blist, const int *tpre, int nact, int p,
-                                                  int jl) {
+__global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blist, const int *tpre, int nact, int jl) {
   // block of this workgroup: largest i with tpre[i] <= blockIdx.x
   int lo = 0, hi = nact - 1;
   const int g = blockIdx.x;
@@ -371,7 +379,8 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
     const int mid = (lo + hi + 1) >> 1;
     if (tpre[mid] <= g) lo = mid; else hi = mid - 1;
   }
-  const BlkDesc bd = a.bd[blist[lo]];
+  const int ent = blist[lo], p = ent >> COL_PANEL_SHIFT;
+  const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
   const int tl = g - tpre[lo];
   const int n = (int)bd.n;
   const int i0 = p * NB;
@@ -443,62 +452,91 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
 }
 
 // ------------------------------------------------------------------------------------------
-// 2b. trailing update A22 -= V W^T + W V^T (full square), 64x64 tiles
+// 2c. tail of the reduction: the trailing matrix of order m <= TAIL left after the last panel
+//     (the whole block when n <= TAIL) is reduced column by column (dsytd2, lower) in LDS by one
+//     workgroup, with the reflectors, tau, d and e stored as the column kernels store them.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_syr2k(SyevArgs<T> a, const int2 *items, int p) {
-  const int2 it = items[blockIdx.x];
-  const BlkDesc bd = a.bd[it.x];
+__global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *blist) {
+  const int ent = blist[blockIdx.x], p = ent >> COL_PANEL_SHIFT;
+  const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
   const int n = (int)bd.n;
   const int i0 = p * NB;
-  const int mt = n - i0 - NB;                 // trailing order
-  const int td = (mt + 63) / 64;
-  const int tr = it.y % td, tc = it.y / td;
-  const T *Vp = a.A + bd.offA + (size_t)i0 * n + (i0 + NB);  // V(r,k) = Vp[k*n + r]
-  const T *Wq = a.Wp + bd.offWp + NB;                        // W(r,k) = Wq[k*n + r]
-  T *C = a.A + bd.offA + (size_t)(i0 + NB) * n + (i0 + NB);
-  __shared__ T Vr[NB][64], Wr[NB][64], Vc[NB][64], Wc[NB][64];
+  const int m = n - i0;
+  T *A = a.A + bd.offA + (size_t)i0 * n + i0;   // A(r,c) = A[c*n + r]
+  double *d = a.d + bd.offV + i0;
+  double *e = a.e + bd.offV + i0;
+  T *tau = a.tau + bd.offV + i0;
+  constexpr int LD = TAIL + 1;
+  constexpr int NW = NT_PANEL / WAVE;
+  __shared__ T S[TAIL * LD];        // S(r,c) at S[c*LD + r], full symmetric
+  __shared__ T vv[TAIL], yy[TAIL];
+  __shared__ T red[NW];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < NB * 64; idx += 256) {
-    const int k = idx / 64, rr = idx % 64;
-    const int row = tr * 64 + rr, col = tc * 64 + rr;
-    Vr[k][rr] = row < mt ? Vp[(size_t)k * n + row] : (T)0;
-    Wr[k][rr] = row < mt ? Wq[(size_t)k * n + row] : (T)0;
-    Vc[k][rr] = col < mt ? Vp[(size_t)k * n + col] : (T)0;
-    Wc[k][rr] = col < mt ? Wq[(size_t)k * n + col] : (T)0;
+  for (int idx = tid; idx < m * m; idx += NT_PANEL) {
+    const int r = idx % m, c = idx / m;
+    if (r >= c) {
+      const T x = A[(size_t)c * n + r];
+      S[c * LD + r] = x;
+      S[r * LD + c] = x;
+    }
   }
   __syncthreads();
-  const int tx = tid & 15, ty = tid >> 4;
-  T acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0;
-#pragma unroll 4
-  for (int k = 0; k < NB; k++) {
-    T av[4], aw[4], bv[4], bw[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      av[i] = Vr[k][tx + 16 * i];
-      aw[i] = Wr[k][tx + 16 * i];
-      bv[i] = Vc[k][ty + 16 * i];
-      bw[i] = Wc[k][ty + 16 * i];
+  for (int j = 0; j + 1 < m; j++) {
+    // reflector annihilating S(j+2:m, j) (dlarfg)
+    const int r1 = j + 1;
+    T part = 0;
+    for (int r = r1 + 1 + tid; r < m; r += NT_PANEL) part += S[j * LD + r] * S[j * LD + r];
+    const T xn2 = block_sum<NT_PANEL>(part, red);
+    const T alpha = S[j * LD + r1];
+    T beta, tv, scal = 1;
+    if (xn2 == (T)0) {
+      tv = 0;
+      beta = alpha;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tv = (beta - alpha) / beta;
+      scal = (T)1 / (alpha - beta);
     }
+    for (int r = r1 + tid; r < m; r += NT_PANEL) {
+      const T x = r == r1 ? (T)1 : S[j * LD + r] * scal;
+      vv[r] = x;
+      A[(size_t)j * n + r] = x;
+    }
+    if (tid == 0) {
+      d[j] = (double)S[j * LD + j];
+      e[j] = (double)beta;
+      tau[j] = tv;
+    }
+    __syncthreads();
+    if (tv != (T)0) {
+      // y = tau S22 v over the rows [r1, m): 16 threads per row
+      const int mt = m - r1;
+      T dp = 0;
+      for (int base = 0; base < mt; base += NT_PANEL / 16) {
+        const int rr = base + tid / 16, sub = tid % 16;
+        T y = 0;
+        if (rr < mt)
+          for (int c = sub; c < mt; c += 16) y += S[(r1 + c) * LD + r1 + rr] * vv[r1 + c];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bw[j] + aw[i] * bv[j];
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int col = tc * 64 + ty + 16 * j;
-    if (col >= mt) continue;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int row = tr * 64 + tx + 16 * i;
-      if (row < mt) C[(size_t)col * n + row] -= acc[i][j];
+        for (int o = 8; o > 0; o >>= 1) y += __shfl_xor(y, o, WAVE);
+        if (rr < mt && sub == 0) {
+          y *= tv;
+          yy[r1 + rr] = y;
+          dp += y * vv[r1 + rr];
+        }
+      }
+      const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
+      // w = y + a2 v; S22 -= v w^T + w v^T
+      for (int idx = tid; idx < mt * mt; idx += NT_PANEL) {
+        const int rr = r1 + idx % mt, cc = r1 + idx / mt;
+        const T wr = yy[rr] + a2 * vv[rr], wc = yy[cc] + a2 * vv[cc];
+        S[cc * LD + rr] -= vv[rr] * wc + wr * vv[cc];
+      }
+      __syncthreads();
     }
   }
+  if (tid == 0 && m >= 1) d[m - 1] = (double)S[(m - 1) * LD + (m - 1)];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1102,155 +1140,6 @@ __global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<T> a, const int2 *item
 }
 
 // ------------------------------------------------------------------------------------------
-// 4a. compact-WY T factor of panel p (dlarft, forward, columnwise)
-// ------------------------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_bt_tfactor(SyevArgs<T> a, const int2 *items) {
-  const int2 it = items[blockIdx.x];
-  const BlkDesc bd = a.bd[it.x];
-  const int n = (int)bd.n, p = it.y, i0 = p * NB;
-  const int m = n - i0;
-  const int w = (m - 1 <= NB) ? m - 1 : NB;
-  const T *A = a.A + bd.offA;  // reflector jj of this panel: column i0+jj, unit at row i0+jj+1
-  const T *tau = a.tau + bd.offV + i0;
-  T *Tp = a.Tf + bd.offT + (size_t)p * NB * NB;  // Tp(r,c) = Tp[c*NB + r]
-  __shared__ T G[NB][NB + 1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  // G(x,y) = y_x^T y_y for x < y
-  for (int pr = wid; pr < w * w; pr += 4) {
-    const int x = pr % w, y = pr / w;
-    if (x >= y) continue;
-    const T *cx = A + (size_t)(i0 + x) * n, *cy = A + (size_t)(i0 + y) * n;
-    T s = 0;
-    for (int r = i0 + y + 2 + lane; r < n; r += 64) s += cx[r] * cy[r];
-    s = wave_sum(s);
-    if (lane == 0) G[x][y] = s + cx[i0 + y + 1];  // row i0+y+1: y_y = 1, y_x stored
-  }
-  __syncthreads();
-  // recurrence, one column at a time; threads x < i compute T(x, i)
-  __shared__ T Ts[NB][NB + 1];
-  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) Ts[idx % NB][idx / NB] = 0;
-  __syncthreads();
-  for (int i = 0; i < w; i++) {
-    const T ti = tau[i];
-    if (threadIdx.x < i) {
-      const int x = threadIdx.x;
-      T s = 0;
-      for (int c = x; c < i; c++) s += Ts[x][c] * G[c][i];
-      Ts[x][i] = -ti * s;
-    }
-    if (threadIdx.x == 0) Ts[i][i] = ti;
-    __syncthreads();
-  }
-  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) Tp[idx] = Ts[idx % NB][idx / NB];
-}
-
-// ------------------------------------------------------------------------------------------
-// 4b. V = Q_H Z on a strip of bt_cols(n) columns, Z <- (I - Y T Y^T) Z for the panels from last to
-//     first. The strip stays in LDS for the whole sweep; the reflectors stream through 32-row chunks.
-// ------------------------------------------------------------------------------------------
-template <typename T>
-__host__ __device__ constexpr int bt_cols(int n) {
-  return (size_t)n * 16 * sizeof(T) <= 131072 ? 16 : ((size_t)n * 8 * sizeof(T) <= 131072 ? 8 : 4);
-}
-constexpr int BT_LDS = 131072;
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_bt_apply(SyevArgs<T> a, const int2 *items) {
-  __shared__ T Zl[BT_LDS / sizeof(T)];   // strip, column-major with leading dimension n
-  __shared__ T Ys[32][NB + 1];           // [row][reflector]
-  __shared__ T Ts[NB][NB + 1];
-  __shared__ T W1[NB][17], W2[NB][17];
-  const int2 it = items[blockIdx.x];
-  const BlkDesc bd = a.bd[it.x];
-  const int n = (int)bd.n;
-  const int cw = bt_cols<T>(n);
-  const int c0 = it.y * cw;
-  const int ncol = min(cw, n - c0);
-  const T *A = a.A + bd.offA;
-  T *Z = a.Zt + bd.offZ;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < n * cw; idx += 256) {
-    const int c = idx / n, r = idx - c * n;
-    Zl[idx] = c < ncol ? Z[(size_t)(c0 + c) * n + r] : (T)0;
-  }
-  const int x = tid & 31, cg = tid >> 5;        // W1 phase: reflector x, columns cg, cg+8
-  const int rr = tid & 31, cq = tid >> 5;       // update phase: row rr of the chunk, columns cq, cq+8
-  for (int p = bd.P - 1; p >= 0; p--) {
-    const int i0 = p * NB, m = n - i0;
-    const int w = (m - 1 <= NB) ? m - 1 : NB;
-    const T *Tp = a.Tf + bd.offT + (size_t)p * NB * NB;
-    __syncthreads();
-    for (int idx = tid; idx < NB * NB; idx += 256) Ts[idx % NB][idx / NB] = Tp[idx];
-    T acc0 = 0, acc1 = 0;
-    for (int r0 = i0 + 1; r0 < n; r0 += 32) {
-      __syncthreads();
-      for (int idx = tid; idx < 32 * NB; idx += 256) {
-        const int q = idx & 31, y = idx >> 5;
-        const int r = r0 + q;
-        T v = 0;
-        if (r < n && y < w) {
-          const int unit = i0 + y + 1;
-          v = r == unit ? (T)1 : (r > unit ? A[(size_t)(i0 + y) * n + r] : (T)0);
-        }
-        Ys[q][y] = v;
-      }
-      __syncthreads();
-      const int rmax = min(32, n - r0);
-      const T *z0 = Zl + (size_t)cg * n + r0, *z1 = Zl + (size_t)(cg + 8) * n + r0;
-      if (cg + 8 < cw) {
-        for (int q = 0; q < rmax; q++) {
-          const T yv = Ys[q][x];
-          acc0 += yv * z0[q];
-          acc1 += yv * z1[q];
-        }
-      } else if (cg < cw) {
-        for (int q = 0; q < rmax; q++) acc0 += Ys[q][x] * z0[q];
-      }
-    }
-    if (cg < cw) W1[x][cg] = acc0;
-    if (cg + 8 < cw) W1[x][cg + 8] = acc1;
-    __syncthreads();
-    // W2 = T W1 (T upper triangular)
-    for (int idx = tid; idx < NB * cw; idx += 256) {
-      const int y = idx % NB, c = idx / NB;
-      T sum = 0;
-      for (int q = y; q < NB; q++) sum += Ts[y][q] * W1[q][c];
-      W2[y][c] = sum;
-    }
-    // Z -= Y W2
-    for (int r0 = i0 + 1; r0 < n; r0 += 32) {
-      __syncthreads();
-      for (int idx = tid; idx < 32 * NB; idx += 256) {
-        const int q = idx & 31, y = idx >> 5;
-        const int r = r0 + q;
-        T v = 0;
-        if (r < n && y < w) {
-          const int unit = i0 + y + 1;
-          v = r == unit ? (T)1 : (r > unit ? A[(size_t)(i0 + y) * n + r] : (T)0);
-        }
-        Ys[q][y] = v;
-      }
-      __syncthreads();
-      const int r = r0 + rr;
-      if (r < n) {
-        for (int c = cq; c < cw; c += 8) {
-          T sum = 0;
-#pragma unroll 8
-          for (int y = 0; y < NB; y++) sum += Ys[rr][y] * W2[y][c];
-          Zl[(size_t)c * n + r] -= sum;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < n * ncol; idx += 256) {
-    const int c = idx / n, r = idx - c * n;
-    Z[(size_t)(c0 + c) * n + r] = Zl[idx];
-  }
-}
-
-// ------------------------------------------------------------------------------------------
 // 5. results back into the caller's arenas
 // ------------------------------------------------------------------------------------------
 template <typename T>
@@ -1589,21 +1478,20 @@ int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, dou
 // ---------------------------------------------------------------------------------------
 struct SyevPlan {
   std::vector<BlkDesc> bd;
-  int64_t sumn = 0, sumn2 = 0, sumWp = 0, sumT = 0;
+  int64_t sumn = 0, sumn2 = 0, sumWp = 0;
   int Pmax = 0, Lmax = 0, nmax = 0;
   // work lists, all packed into one int array; offsets in ints
   std::vector<int> ibuf;
   size_t symm_off = 0, symm_cnt = 0;
-  struct ColLaunch { int p, jl, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };
+  struct ColLaunch { int p, jl, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };   // p: panel slot
   std::vector<ColLaunch> cols;
-  std::vector<size_t> upd_off, upd_cnt;
   int64_t sumYp = 0;
   size_t all_off = 0;  // list 0..nblk-1
   std::vector<size_t> mrg_off, mrg_cnt, gemm_off, gemm_cnt;
   std::vector<size_t> ch_off, ch_cnt, fc_off, fc_cnt;  // 256-wide chunks and FILLC-wide column chunks
   std::vector<size_t> rc_off, rc_cnt;                  // RCH-wide root chunks
   size_t mrg_max = 0;
-  size_t tf_off = 0, tf_cnt = 0, bt_off = 0, bt_cnt = 0, cp_off = 0, cp_cnt = 0;
+  size_t cp_off = 0, cp_cnt = 0, tail_off = 0, tail_cnt = 0;
   std::vector<int> mcap;
   std::vector<GemmDesc> gd;   // products, filled once the arena pointers are known
   // FP64 back-transformation
@@ -1616,17 +1504,16 @@ struct SyevPlan {
   std::vector<size_t> s2_items, s2_cnt;   // FP64 trailing rank-2k updates per panel
 };
 
-static int panels_of(int64_t n) {
-  if (n <= 1) return n == 1 ? 1 : 0;
-  int P = 1;
-  while (n - (int64_t)NB * (P - 1) - 1 > NB) P++;
-  return P;
+// Panels reduced by the column kernels: those that start with more than TAIL trailing rows. The
+// trailing matrix of order <= TAIL that remains is reduced by k_tail.
+static int main_panels(int64_t n) {
+  return n > TAIL ? (int)((n - TAIL + NB - 1) / NB) : 0;
 }
 
 static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *off_H, const int64_t *off_w,
                        bool fp64_inplace, int elem_bytes) {
   pl.bd.resize(nblk);
-  int64_t zo = 0, vo = 0, wo = 0, to = 0, ao = 0;
+  int64_t zo = 0, vo = 0, wo = 0, ao = 0;
   for (int b = 0; b < nblk; b++) {
     BlkDesc &d = pl.bd[b];
     d.n = n[b];
@@ -1634,8 +1521,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     d.offA = fp64_inplace ? off_H[b] : ao; ao += n[b] * n[b];
     d.offV = vo; vo += n[b];
     d.offWp = wo; wo += n[b] * NB;
-    d.P = panels_of(n[b]);
-    d.offT = to; to += (int64_t)d.P * NB * NB;
+    d.P = main_panels(n[b]);
     d.offYp = pl.sumYp; pl.sumYp += n[b] * ((n[b] + 63) / 64);
     d.offDt = pl.sumYp; pl.sumYp += 64 * ((n[b] + 63) / 64);
     d.offH = off_H[b];
@@ -1645,7 +1531,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     pl.Lmax = std::max(pl.Lmax, d.L);
     pl.nmax = std::max(pl.nmax, (int)n[b]);
   }
-  pl.sumn2 = zo; pl.sumn = vo; pl.sumWp = wo; pl.sumT = to;
+  pl.sumn2 = zo; pl.sumn = vo; pl.sumWp = wo;
   std::vector<int> &I = pl.ibuf;
   I.clear();
   auto push2 = [&](int x, int y) { I.push_back(x); I.push_back(y); };
@@ -1662,35 +1548,37 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     for (int64_t q = 0; q < t * (t + 1) / 2; q++) push2(b, (int)q);
   }
   pl.symm_cnt = (I.size() - pl.symm_off) / 2;
-  // reduced columns (two launches each) and trailing updates
+  // reduced columns (two launches each) and trailing updates, on a common sequence of Pmax panel
+  // slots: block b works on panel panel_in_slot(q, P_b) in slot q. Work-list entries of the column
+  // kernels carry the block and its panel (col_entry).
   pl.cols.clear();
-  pl.upd_off.assign(pl.Pmax, 0); pl.upd_cnt.assign(pl.Pmax, 0);
-  for (int p = 0; p < pl.Pmax; p++) {
+  for (int q = 0; q < pl.Pmax; q++) {
+    auto panel = [&](int b) { return panel_in_slot(q, pl.bd[b].P); };
     auto width = [&](int b) {
-      const int64_t m = n[b] - (int64_t)NB * p;
+      const int64_t m = n[b] - (int64_t)NB * panel(b);
       return (int)(m - 1 <= NB ? m - 1 : NB);
     };
     int wmax = 0;
-    for (int b : order) if (pl.bd[b].P > p) wmax = std::max(wmax, width(b));
+    for (int b : order) if (panel(b) >= 0) wmax = std::max(wmax, width(b));
     for (int jl = 0; jl <= wmax; jl++) {
       SyevPlan::ColLaunch c;
-      c.p = p; c.jl = jl;
+      c.p = q; c.jl = jl;
       c.prep_off = I.size();
       c.rows = 0;
       for (int b : order)
-        if (pl.bd[b].P > p && jl <= width(b)) {
-          I.push_back(b);
-          c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * p - jl));
+        if (panel(b) >= 0 && jl <= width(b)) {
+          I.push_back(col_entry(b, panel(b)));
+          c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * panel(b) - jl));
         }
       c.prep_cnt = I.size() - c.prep_off;
       c.symv_off = I.size();
-      for (int b : order) if (pl.bd[b].P > p && jl < width(b)) I.push_back(b);
+      for (int b : order) if (panel(b) >= 0 && jl < width(b)) I.push_back(col_entry(b, panel(b)));
       c.nact = I.size() - c.symv_off;
       c.tpre_off = I.size();
       int64_t tiles = 0;
-      for (size_t q = 0; q < c.nact; q++) {
-        const int b = I[c.symv_off + q];
-        const int64_t t = n[b] - (int64_t)NB * p - jl - 1;
+      for (size_t k = 0; k < c.nact; k++) {
+        const int b = I[c.symv_off + k] & COL_BLK_MASK;
+        const int64_t t = n[b] - (int64_t)NB * panel(b) - jl - 1;
         const int64_t T = (t + 63) / 64;
         I.push_back((int)tiles);
         tiles += T * (T + 1) / 2 + (jl > 0 ? T : 0);   // + one dot-product workgroup per segment
@@ -1698,15 +1586,12 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
       c.tiles = (size_t)tiles;
       pl.cols.push_back(c);
     }
-    pl.upd_off[p] = I.size();
-    for (int b : order)
-      if (pl.bd[b].P > p + 1) {
-        const int64_t mt = n[b] - (int64_t)NB * (p + 1);
-        const int64_t td = (mt + 63) / 64;
-        for (int64_t t = 0; t < td * td; t++) push2(b, (int)t);
-      }
-    pl.upd_cnt[p] = (I.size() - pl.upd_off[p]) / 2;
   }
+  // tail of every block (one k_tail launch after the column launches)
+  pl.tail_off = I.size();
+  for (int b : order)
+    if (n[b] >= 1) I.push_back(col_entry(b, pl.bd[b].P));
+  pl.tail_cnt = I.size() - pl.tail_off;
   // divide and conquer merges, deepest first
   pl.mrg_off.assign(pl.Lmax, 0); pl.mrg_cnt.assign(pl.Lmax, 0);
   pl.gemm_off.assign(pl.Lmax, 0); pl.gemm_cnt.assign(pl.Lmax, 0);
@@ -1747,19 +1632,6 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
       for (int c = 0; c < (sizes_here[mi] + FILLC - 1) / FILLC; c++) push2((int)mi, c);
     pl.fc_cnt[depth] = (I.size() - pl.fc_off[depth]) / 2;
   }
-  // compact-WY factors and back-transformation strips
-  pl.tf_off = I.size();
-  for (int b : order)
-    if (n[b] >= 2)
-      for (int p = 0; p < pl.bd[b].P; p++) push2(b, p);
-  pl.tf_cnt = (I.size() - pl.tf_off) / 2;
-  pl.bt_off = I.size();
-  for (int b : order)
-    if (n[b] >= 3) {
-      const int cw = elem_bytes == 4 ? bt_cols<float>((int)n[b]) : bt_cols<double>((int)n[b]);
-      for (int s = 0; s < (n[b] + cw - 1) / cw; s++) push2(b, s);
-    }
-  pl.bt_cnt = (I.size() - pl.bt_off) / 2;
   // FP64 back-transformation: super-panel offsets, Y columns and T factors
   pl.toff.assign(nblk, 0);
   pl.sumTb = 0; pl.sumW2 = 0; pl.Smax = 0;
@@ -1831,10 +1703,11 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
   std::vector<int> &I = pl.ibuf;
   // trailing updates A22 -= V W^T + W V^T after every panel that is not a block's last
   pl.s2_items.assign(pl.Pmax, 0); pl.s2_cnt.assign(pl.Pmax, 0);
-  for (int p = 0; p < pl.Pmax; p++) {
+  for (int q = 0; q < pl.Pmax; q++) {   // panel slot q, as in build_plan
     const size_t d0 = gd.size();
     for (const BlkDesc &d : pl.bd) {
-      if (d.P <= p + 1) continue;
+      const int p = panel_in_slot(q, d.P);
+      if (p < 0) continue;
       const int n = (int)d.n, i0 = p * NB, mt = n - i0 - NB;
       GemmDescT<T> g{};
       T *Ab = Abase + d.offA;
@@ -1847,7 +1720,7 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
     }
     // only the tiles on or below the first tile super-diagonal: the reduction reads the lower
     // triangle, and the symmetric product's diagonal 64x64 tiles reach at most 63 columns above it
-    pl.s2_items[p] = I.size();
+    pl.s2_items[q] = I.size();
     for (size_t i = d0; i < gd.size(); i++) {
       const int tm = (gd[i].M + GEMM_TILE - 1) / GEMM_TILE;
       const int tiles = tm * ((gd[i].N + GEMM_TILE - 1) / GEMM_TILE);
@@ -1857,7 +1730,7 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
           I.push_back(k);
         }
     }
-    pl.s2_cnt[p] = (I.size() - pl.s2_items[p]) / 2;
+    pl.s2_cnt[q] = (I.size() - pl.s2_items[q]) / 2;
   }
   // back-transformation products
   const int nblk = (int)pl.bd.size();
@@ -1964,23 +1837,25 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off), 0));
   // 2. reduction
   size_t ci = 0;
-  for (int p = 0; p < pl.Pmax; p++) {
-    for (; ci < pl.cols.size() && pl.cols[ci].p == p; ci++) {
+  for (int q = 0; q < pl.Pmax; q++) {
+    for (; ci < pl.cols.size() && pl.cols[ci].p == q; ci++) {
       const SyevPlan::ColLaunch &c = pl.cols[ci];
       if (c.prep_cnt) {
         if (c.rows <= NT_PANEL)
           NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, st, aT,
-                                                              I + c.prep_off, p, c.jl));
+                                                              I + c.prep_off, c.jl));
         else
           NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, NMAX / NT_PANEL>), grid(c.prep_cnt), dim3(NT_PANEL),
-                                                              0, st, aT, I + c.prep_off, p, c.jl));
+                                                              0, st, aT, I + c.prep_off, c.jl));
       }
       if (c.tiles)
         NRG_TIMED(ctx, st, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, st, aT,
-                                                            I + c.symv_off, I + c.tpre_off, (int)c.nact, p, c.jl));
+                                                            I + c.symv_off, I + c.tpre_off, (int)c.nact, c.jl));
     }
-    launch_gemm<T>(ctx, st, L.dGT, I + pl.s2_items[p], H + pl.s2_items[p], pl.s2_cnt[p]);
+    launch_gemm<T>(ctx, st, L.dGT, I + pl.s2_items[q], H + pl.s2_items[q], pl.s2_cnt[q]);
   }
+  if (pl.tail_cnt)
+    NRG_TIMED(ctx, st, "k_tail", hipLaunchKernelGGL(k_tail<T>, grid(pl.tail_cnt), dim3(NT_PANEL), 0, st, aT, I + pl.tail_off));
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 1], st));
   // 3. divide and conquer
@@ -2068,7 +1943,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   build_plan(pl, nblk, n, off_H, off_w, true, 8);
   Carver cv;
   const size_t oZ0 = cv.take<double>(pl.sumn2), oZ1 = cv.take<double>(pl.sumn2), oS = cv.take<double>(pl.sumn2);
-  const size_t oWp = cv.take<double>(pl.sumWp), oTau = cv.take<double>(pl.sumn), oTf = cv.take<double>(pl.sumT);
+  const size_t oWp = cv.take<double>(pl.sumWp), oTau = cv.take<double>(pl.sumn);
   const size_t od = cv.take<double>(pl.sumn), oe = cv.take<double>(pl.sumn), ol = cv.take<double>(pl.sumn);
   const size_t osc = cv.take<double>(nblk), odscr = cv.take<double>(DS_COUNT * pl.sumn), oiscr = cv.take<int>(IS_COUNT * pl.sumn);
   const size_t ovb = cv.take<double>(pl.sumn), oYp = cv.take<double>(pl.sumYp);
@@ -2101,7 +1976,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   SyevArgs<double> a;
   a.bd = (const BlkDesc *)(ctx->desc.as<char>(oBD));
   a.A = H;
-  a.Wp = ctx->ws.as<double>(oWp); a.tau = ctx->ws.as<double>(oTau); a.Tf = ctx->ws.as<double>(oTf);
+  a.Wp = ctx->ws.as<double>(oWp); a.tau = ctx->ws.as<double>(oTau);
   a.vb = ctx->ws.as<double>(ovb); a.Yp = ctx->ws.as<double>(oYp);
   a.Zt = Z0; a.Z0 = Z0; a.Z1 = Z1; a.S = S;
   a.d = ctx->ws.as<double>(od); a.e = ctx->ws.as<double>(oe); a.lam = ctx->ws.as<double>(ol);
@@ -2176,7 +2051,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   // FP32 workspace
   Carver cf;
   const size_t fA = cf.take<float>(pl.sumn2), fWp = cf.take<float>(pl.sumWp), fTau = cf.take<float>(pl.sumn);
-  const size_t fTf = cf.take<float>(pl.sumT), fvb = cf.take<float>(pl.sumn), fYp = cf.take<float>(pl.sumYp);
+  const size_t fvb = cf.take<float>(pl.sumn), fYp = cf.take<float>(pl.sumYp);
   const size_t fYm = cf.take<float>(pl.sumn2), fYh = cf.take<float>(pl.sumn2), fZt = cf.take<float>(pl.sumn2);
   const size_t fGb = cf.take<float>(pl.sumTb + 1), fTb = cf.take<float>(pl.sumTb + 1), fW2 = cf.take<float>(pl.sumW2);
   if (int r = ctx->ws2.ensure(cf.off)) return r;
@@ -2254,7 +2129,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   aD.Ntot = pl.sumn; aD.flags = (int *)ctx->flags.p; aD.amax = ctx->ws.as<double>(oamax); aD.H = H; aD.w = w;
   SyevArgs<float> aF{};
   aF.bd = dbd; aF.A = A32; aF.Wp = ctx->ws2.as<float>(fWp); aF.tau = ctx->ws2.as<float>(fTau);
-  aF.Tf = ctx->ws2.as<float>(fTf); aF.vb = ctx->ws2.as<float>(fvb); aF.Yp = ctx->ws2.as<float>(fYp);
+  aF.vb = ctx->ws2.as<float>(fvb); aF.Yp = ctx->ws2.as<float>(fYp);
   aF.Ym = ctx->ws2.as<float>(fYm); aF.Yh = ctx->ws2.as<float>(fYh); aF.Zt = ctx->ws2.as<float>(fZt);
   aF.Gb = ctx->ws2.as<float>(fGb); aF.Tb = ctx->ws2.as<float>(fTb);
   aF.Z0 = Z0; aF.Z1 = Z1; aF.S = S; aF.d = aD.d; aF.e = aD.e; aF.lam = aD.lam; aF.scale = aD.scale;
