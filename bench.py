@@ -31,15 +31,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def panels_of(n, nb=32):
-    """Number of 32-column panels of the Householder reduction of an order-n block (the last panel
-    reduces the remaining m - 1 <= 32 columns)."""
-    if n <= 1:
-        return 1 if n == 1 else 0
-    p = 1
-    while n - nb * (p - 1) - 1 > nb:
-        p += 1
-    return p
+TAIL = 128   # trailing order the library reduces inside one workgroup (k_tail)
+
+
+def main_panels(n, nb=32, tail=TAIL):
+    """Panels of a block of order n reduced by the column kernels: those starting with more than
+    `tail` trailing rows (the rest goes to k_tail)."""
+    return -(-(n - tail) // nb) if n > tail else 0
 
 
 def reduction_bytes(sizes, nb=32):
@@ -47,38 +45,31 @@ def reduction_bytes(sizes, nb=32):
     (LAPACK dsytrd/dlatrd, lower), summed over the launches of one call:
 
     k_col_symv, column jl of panel p of a block (trailing order t = m - jl - 1, m = n - 32 p):
-        the lower triangle of the trailing matrix read once, v read, y written: 8 (t (t + 1) / 2 + 2 t)
-    k_col_prep, same column: the panel's V and W columns [0, jl) over the m - jl trailing rows read once,
-        the ceil((m - jl) / 64) SYMV partial sums of the previous column read, the column read and
-        written and W's new column written: 8 (2 jl (m - jl) + ceil((m - jl) / 64) (m - jl) + 3 (m - jl))
+        the lower triangle of the trailing matrix read once, the panel's V and W columns [0, jl)
+        over the t trailing rows read once (for W^T v and V^T v), v read, y written:
+        8 (t (t + 1) / 2 + 2 jl t + 2 t)
+    k_col_prep, same column: the panel's V and W columns [0, jl) over the r = m - jl rows read once,
+        the ceil(r / 64) symmetric-product partials and 64 ceil(r / 64) dot-product partials of the
+        previous column read, the column read and written and W's new column written:
+        8 (2 jl r + ceil(r / 64) (r + 64) + 3 r)
 
     Returns {kernel: (total bytes, launches)}; launch counts follow the library's schedule (one launch
-    per (panel, column) over all blocks still active there)."""
+    per (panel, column) over all blocks still in their column-kernel panels)."""
     sizes = [int(n) for n in sizes]
-    pmax = max(panels_of(n, nb) for n in sizes)
+    pmax = max(main_panels(n, nb) for n in sizes)
     prep_b = symv_b = 0.0
     prep_l = symv_l = 0
     for p in range(pmax):
-        widths = []
-        for n in sizes:
-            if panels_of(n, nb) <= p:
-                continue
-            m = n - p * nb
-            widths.append((m, m - 1 if m - 1 <= nb else nb))
-        wmax = max(w for _, w in widths)
-        for jl in range(wmax + 1):
-            any_prep = any_symv = False
-            for m, w in widths:
-                if jl <= w:
-                    any_prep = True
-                    r = m - jl
-                    prep_b += 8.0 * (2 * jl * r + ((r + 63) // 64) * r + 3 * r)
-                if jl < w:
-                    any_symv = True
+        ms = [n - p * nb for n in sizes if main_panels(n, nb) > p]
+        for jl in range(nb + 1):
+            for m in ms:
+                r = m - jl
+                prep_b += 8.0 * (2 * jl * r + ((r + 63) // 64) * (r + 64) + 3 * r)
+                if jl < nb:
                     t = m - jl - 1
-                    symv_b += 8.0 * (t * (t + 1) / 2.0 + 2 * t)
-            prep_l += any_prep
-            symv_l += any_symv
+                    symv_b += 8.0 * (t * (t + 1) / 2.0 + 2 * jl * t + 2 * t)
+            prep_l += 1
+            symv_l += jl < nb
     return {"k_col_prep": (prep_b, prep_l), "k_col_symv": (symv_b, symv_l)}
 
 
