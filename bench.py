@@ -321,9 +321,14 @@ def bench_recalc(args):
         worst = max(worst, float(np.linalg.norm(co - cf)) / scale)
     mode = args.mode if args.mode in ("fp64", "ozaki") else "ozaki"
     ms = res[mode][0]
-    kname = "k_gemm_f64" if mode == "fp64" else "k_oz_gemm"
-    tms, cnt = kts[mode][kname]
-    ach = flops / (tms / cnt * 1e-3 * 2) / 1e12 if cnt else None   # two launches per call
+    # the product kernels' time per call (two timed calls): both FP64 tile kernels, or the slice kernel
+    if mode == "fp64":
+        kname = " + ".join(k for k in sorted(kts[mode]) if k.startswith("k_gemm"))
+        kms = sum(v[0] for k, v in kts[mode].items() if k.startswith("k_gemm")) / 2
+    else:
+        kname = "k_oz_gemm"
+        kms = kts[mode][kname][0] / 2
+    ach = flops / (kms * 1e-3) / 1e12 if kms > 0 else None
     roof = {"bound": "mfma", "kernel": kname, "achieved": round(ach, 3), "unit": "TFLOP/s", "traffic": None}
     if mode == "fp64":
         roof.update(peak=FP64_PEAK_TFLOPS, frac=round(ach / FP64_PEAK_TFLOPS, 4))
