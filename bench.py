@@ -454,6 +454,87 @@ def bench_fdm(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_dmnrg(args):
+    """DM-NRG spectral weights + log binning of one synthetic step (nrg_dmnrg_bin); value = elements/s."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, FdmPair, BinGrid, workloads
+    pairs = workloads.dmnrg_step(npairs=args.npairs)
+    nel = sum(p["K_i"] * p["K_j"] for p in pairs)
+    flops = workloads.dmnrg_flops(pairs)
+    dev = torch.device("cuda:0")
+    grid = BinGrid(-12.0, 1000.0, 14001, 1e-16)
+    emin, emax = 2.0, 8.0   # the window of a step of scale 1 with goodE = 2, Lambda = 2
+    keep, structs = [], []
+    for p in pairs:
+        t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+        keep.append(t)
+        structs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], t["E_i"].data_ptr(), t["E_j"].data_ptr(),
+                               t["op"].data_ptr(), t["op"].data_ptr(), p["dim_i"], t["rho_i"].data_ptr(), p["K_i"],
+                               t["rho_j"].data_ptr(), p["K_j"], 1.0))
+    bins = [torch.zeros(grid.nbins, dtype=torch.float64, device=dev) for _ in range(2)]
+    ctx = Context(0)
+
+    def step():
+        for b in bins:
+            b.zero_()
+        ctx.dmnrg_bin(structs, grid, emin, emax, -1, *bins)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log(f"dmnrg step: {len(pairs)} pairs, {nel / 1e6:.2f} M kept x kept elements, {flops / 1e9:.2f} GFLOP of products, "
+        f"{ms:.3f} ms per call; " + ", ".join(f"{k} {v[0] / 2:.3f} ms" for k, v in kt.items()))
+    gemm_ms = sum(v[0] for k, v in kt.items() if k.startswith("k_gemm")) / 2
+    bin_ms = kt["k_dmnrg_bin"][0] / 2 if "k_dmnrg_bin" in kt else 0.0
+    if gemm_ms >= bin_ms:
+        ach = flops / (gemm_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": " + ".join(sorted(k for k in kt if k.startswith("k_gemm"))),
+                "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None}
+    else:
+        ach = 32.0 * nel / (bin_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_dmnrg_bin", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "note": "algorithmic bytes = 32 B per element (op1, op2 and the two products)"}
+    cpu = None
+    if not args.no_cpu:
+        # the reference's loops as they stand (inner sums per element), one core, on the smallest pairs
+        order = sorted(range(len(pairs)), key=lambda i: pairs[i]["K_i"] * pairs[i]["K_j"])
+        hs, hold, el = [], [], 0
+        for i in order:
+            p = pairs[i]
+            arrs = {k: np.ascontiguousarray(p[k]) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+            hold.append(arrs)
+            hs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], arrs["E_i"].ctypes.data,
+                              arrs["E_j"].ctypes.data, arrs["op"].ctypes.data, arrs["op"].ctypes.data, p["dim_i"],
+                              arrs["rho_i"].ctypes.data, p["K_i"], arrs["rho_j"].ctypes.data, p["K_j"], 1.0))
+            el += p["K_i"] * p["K_j"]
+            if len(hs) >= len(pairs) // 2:
+                break
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 5000:
+            oracle.dmnrg_bin(hs, grid, emin, emax, -1, grid.nbins)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * el / dt / 1e6, 3), "unit": "Melements/s", "cores": 1, "kind": "port",
+               "sample": f"the {len(hs)} smallest pairs ({el / 1e6:.2f} M elements) {reps} times, the reference's "
+                         f"loops with per-element inner sums + serial binning, {dt:.1f} s"}
+    out = {"metric": "DM-NRG spectral weights + log binning, elements per second", "value": round(nel / ms / 1e3, 2),
+           "unit": "Melements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"one DM-NRG step, {len(pairs)} subspace pairs, kept dims log-uniform [16,1000], "
+                                  "window [2, 8], 1000 bins/decade over 14 decades, seed 20261208",
+                      "elements": nel, "product_flops": flops},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_broaden(args):
     """Broadening of one binned spectrum (1000 bins/decade over 14 decades, trimmed) onto the standard
     460-point mesh (nrg_broaden); value = BR_NEW evaluations per second."""
@@ -603,7 +684,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm"], default="eig")
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg"], default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--npairs", type=int, default=48)
@@ -615,7 +696,7 @@ def main():
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
-     "dm": bench_dm}[args.workload](args)
+     "dm": bench_dm, "dmnrg": bench_dmnrg}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -146,6 +146,21 @@ int nrg_fdm_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_b
                 double *ls_neg, void *stream);
 
 /*
+ * DM-NRG spectral weights of all subspace pairs of one step, binned: replaces the pair loop of
+ * BaseSpectrum::calc (c++/algo.hpp:55-67) driving Algo_DMNRG::calc (c++/algo_DMNRG.hpp:30-52)
+ * and ChainBinning::add (c++/spectrum.hpp:19-28). Same pair descriptors as nrg_fdm_bin, except that
+ * E_i, E_j are the abs_zero energies (c++/eigen.hpp:44) and rho_i, rho_j the DM-NRG density
+ * matrices of the step (covering the kept states). For every kept m of Ii and kept j of Ij, at
+ * energy E = E_j[j] - E_i[m], when emin <= |E| <= emax:
+ *   weight = (op2 rho_i^T)(j,m) op1(j,m) + (-sign) (rho_j op1)(j,m) op2(j,m)
+ * (sums over the kept states only), times the pair factor. emin, emax: step.scale() * P.getEmin(),
+ * step.scale() * P.getEmax(). Accumulates (+=) into out_pos / out_neg (nbins doubles each); the
+ * N/N+2 merge into the running spectrum (SpectrumRealFreq::mergeNN2) is left to the caller.
+ */
+int nrg_dmnrg_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin,
+                  double emax, int sign, double *out_pos, double *out_neg, void *stream);
+
+/*
  * Broadening of a binned spectrum onto an energy mesh: replaces SpectrumRealFreq::continuous
  * (c++/spectrum.hpp:180-209) with the kernel BR_NEW (c++/spectral.hpp:46-71, log-Gaussian of
  * width alpha blended into a Gaussian of width omega0 below |E| = omega0).

@@ -18,7 +18,7 @@ NRG_GEMM_FP64 = 0
 NRG_GEMM_OZAKI = 1
 
 EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
-            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
+            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
             "nrg_kernel_time", "nrg_kernel_names"]
 
 
@@ -70,6 +70,9 @@ def lib():
     L.nrg_fdm_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
                               ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, vp]
     L.nrg_fdm_bin.restype = ctypes.c_int
+    L.nrg_dmnrg_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
+                                ctypes.c_double, ctypes.c_int, vp, vp, vp]
+    L.nrg_dmnrg_bin.restype = ctypes.c_int
     L.nrg_broaden.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_double,
                               ctypes.c_double, vp, vp, vp]
     L.nrg_broaden.restype = ctypes.c_int
@@ -158,6 +161,15 @@ class Context:
                               _stream_ptr(stream))
         if r != 0:
             raise NrgError(r, "nrg_fdm_bin")
+
+    def dmnrg_bin(self, pairs, grid, emin, emax, sign, out_pos, out_neg, stream=None):
+        """nrg_dmnrg_bin: pairs carry abs_zero energies and the DM-NRG density matrices."""
+        arr = (FdmPair * max(len(pairs), 1))(*pairs)
+        r = lib().nrg_dmnrg_bin(self._h, arr, len(pairs), ctypes.byref(grid), float(emin), float(emax), int(sign),
+                                ctypes.c_void_p(out_pos.data_ptr()), ctypes.c_void_p(out_neg.data_ptr()),
+                                _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_dmnrg_bin")
 
     def broaden(self, pos_e, pos_w, neg_e, neg_w, mesh, alpha, omega0, out_pos, out_neg, stream=None):
         """nrg_broaden on device tensors (float64)."""

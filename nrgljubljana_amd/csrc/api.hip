@@ -217,6 +217,21 @@ int nrg_fdm_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_b
   }
 }
 
+int nrg_dmnrg_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin,
+                  double emax, int sign, double *out_pos, double *out_neg, void *stream) {
+  if (!ctx) {
+    set_error("nrg_dmnrg_bin: NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return dmnrg_run(ctx, pairs, npairs, grid, emin, emax, sign, out_pos, out_neg, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("nrg_dmnrg_bin: ") + e.what());
+    return NRG_E_HIP;
+  }
+}
+
 int nrg_broaden(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
                 const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
                 double *out_pos, double *out_neg, void *stream) {

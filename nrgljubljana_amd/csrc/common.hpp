@@ -129,6 +129,8 @@ int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, dou
 int gemm3_run(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, int precision, hipStream_t st);
 int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf,
             int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, hipStream_t st);
+int dmnrg_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin, double emax,
+              int sign, double *out_pos, double *out_neg, hipStream_t st);
 int broaden_run(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
                 const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
                 double *out_pos, double *out_neg, hipStream_t st);

@@ -13,6 +13,10 @@
 // every pair in 64 x 64 tiles (Boltzmann factors of the tile's states in LDS, one log10 per element
 // shared by both halves) and scatters into replicated bins with FP64 atomics, which one more launch
 // adds into the caller's arrays.
+//
+// nrg_dmnrg_bin (dmnrg_run, k_dmnrg_bin) does the same for Algo_DMNRG::calc (c++/algo_DMNRG.hpp:30-52):
+// the kept x kept elements of every pair, with the two density-matrix products formed on the matrix
+// cores and an energy window instead of Boltzmann factors.
 #include "gemm.hpp"
 #include <cmath>
 #include <cstring>
@@ -36,15 +40,36 @@ struct FdmArgs {
 };
 
 // Bins::add + Bins::add_std (c++/bins.hpp:103-132) with the energy already made non-negative
-__device__ __forceinline__ void bin_add(const FdmArgs &a, double *bins, double energy, double w, int64_t idx,
-                                        double rem, bool low) {
-  if (w == 0.0 || fabs(w) < a.discard * energy) return;
+__device__ __forceinline__ void bin_add(double discard, double *bins, double energy, double w, int64_t idx, double rem,
+                                        bool low) {
+  if (w == 0.0 || fabs(w) < discard * energy) return;
   if (low) {
     atomicAdd(bins + idx, w);
     return;
   }
   atomicAdd(bins + idx, (1.0 - rem) * w);
   atomicAdd(bins + idx + 1, rem * w);
+}
+
+// Bin position of a non-negative energy (Bins::add_std): below zero_epsilon = 1e-14 (c++/bins.hpp:46)
+// or below the grid -> bin 0, above it -> the last bin (low = true: the whole weight in bin idx);
+// otherwise split between idx and idx + 1 with weights 1 - rem and rem.
+__device__ __forceinline__ void bin_pos(double energy, double log10emin, double bpd, int64_t nbins, int64_t &idx,
+                                        double &rem, bool &low) {
+  idx = 0;
+  rem = 0.0;
+  low = true;
+  if (energy < 1e-14) return;
+  const double x = (log10(energy) - log10emin) * bpd;
+  const double ip = floor(x);
+  if (ip < 0) return;
+  if (ip >= (double)(nbins - 1)) {
+    idx = nbins - 1;
+    return;
+  }
+  idx = (int64_t)ip;
+  rem = x - ip;
+  low = false;
 }
 
 // The scatter goes to FDM_COPIES replicas of the four histograms (workgroup w adds into replica
@@ -57,16 +82,17 @@ __global__ __launch_bounds__(256) void k_fdm_zero(double *p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0;
 }
 
-// replica c of histogram h (0 gt_pos, 1 gt_neg, 2 ls_pos, 3 ls_neg) at priv + (c * 4 + h) * nbins
-__global__ __launch_bounds__(256) void k_fdm_reduce(const double *priv, int64_t nbins, double *gt_pos, double *gt_neg,
-                                                    double *ls_pos, double *ls_neg) {
+// replica c of histogram h (< nh) at priv + (c * nh + h) * nbins; FDM: h = 0 gt_pos, 1 gt_neg,
+// 2 ls_pos, 3 ls_neg; DM-NRG: 0 pos, 1 neg
+__global__ __launch_bounds__(256) void k_fdm_reduce(const double *priv, int64_t nbins, int nh, double *o0, double *o1,
+                                                    double *o2, double *o3) {
   const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int h = blockIdx.y;
   if (b >= nbins) return;
   double s = 0.0;
 #pragma unroll 8
-  for (int c = 0; c < FDM_COPIES; c++) s += priv[((size_t)c * 4 + h) * nbins + b];
-  double *out = h == 0 ? gt_pos : h == 1 ? gt_neg : h == 2 ? ls_pos : ls_neg;
+  for (int c = 0; c < FDM_COPIES; c++) s += priv[((size_t)c * nh + h) * nbins + b];
+  double *out = h == 0 ? o0 : h == 1 ? o1 : h == 2 ? o2 : o3;
   out[b] += s;
 }
 
@@ -116,25 +142,13 @@ __global__ __launch_bounds__(256) void k_fdm_bin(FdmArgs a, const int2 *items) {
     const double de = Ej[jl] - Ei[il];
     const bool pos = de >= 0.0;
     const double energy = pos ? de : -de;
-    // bin position, shared by both halves (zero_epsilon = 1e-14, c++/bins.hpp:46)
-    int64_t idx = 0;
-    double rem = 0.0;
-    bool low = true;
-    if (energy >= 1e-14) {
-      const double x = (log10(energy) - a.log10emin) * a.bpd;
-      const double ip = floor(x);
-      if (ip < 0) {
-        idx = 0;
-      } else if (ip >= (double)(a.nbins - 1)) {
-        idx = a.nbins - 1;
-      } else {
-        idx = (int64_t)ip;
-        rem = x - ip;
-        low = false;
-      }
-    }
-    bin_add(a, pos ? a.gt_pos : a.gt_neg, energy, wgt, idx, rem, low);
-    bin_add(a, pos ? a.ls_pos : a.ls_neg, energy, wls, idx, rem, low);
+    // bin position, shared by both halves
+    int64_t idx;
+    double rem;
+    bool low;
+    bin_pos(energy, a.log10emin, a.bpd, a.nbins, idx, rem, low);
+    bin_add(a.discard, pos ? a.gt_pos : a.gt_neg, energy, wgt, idx, rem, low);
+    bin_add(a.discard, pos ? a.ls_pos : a.ls_neg, energy, wls, idx, rem, low);
   }
 }
 
@@ -250,8 +264,176 @@ int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_g
     NRG_TIMED(ctx, st, "k_fdm_bin", hipLaunchKernelGGL(k_fdm_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st, fa,
                                                        (const int2 *)ctx->desc.as<char>(oI)));
     NRG_TIMED(ctx, st, "k_fdm_reduce", hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((grid->nbins + 255) / 256), 4),
-                                                          dim3(256), 0, st, priv, (int64_t)grid->nbins, gt_pos, gt_neg,
-                                                          ls_pos, ls_neg));
+                                                          dim3(256), 0, st, priv, (int64_t)grid->nbins, 4, gt_pos,
+                                                          gt_neg, ls_pos, ls_neg));
+  }
+  NRG_HIP(hipGetLastError());
+  NRG_HIP(hipEventRecord(ctx->ev[2], st));
+  NRG_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]);
+  ctx->total_ms = ms;
+  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+  ctx->phase_ms[0] = ms;
+  hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
+  ctx->phase_ms[1] = ms;
+  if (ctx->kt.on) ctx->kt.collect();
+  return NRG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// DM-NRG (Algo_DMNRG::calc, c++/algo_DMNRG.hpp:30-52)
+// ------------------------------------------------------------------------------------------
+struct DmnrgPairDev {
+  int64_t K_i, K_j, ld_op;
+  const double *E_i, *E_j, *op1, *op2;
+  const double *pa;   // K_j x K_i row-major: (op2 rho_i^T)(j, m) over the kept states
+  const double *pb;   // K_j x K_i row-major: (rho_j op1)(j, m)
+  double factor;
+};
+
+struct DmnrgArgs {
+  const DmnrgPairDev *pairs;
+  double emin, emax, msign;
+  double log10emin, bpd, discard;
+  int64_t nbins;
+  double *priv;   // FDM_COPIES replicas of (pos, neg)
+};
+
+__global__ __launch_bounds__(256) void k_dmnrg_bin(DmnrgArgs a, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const DmnrgPairDev P = a.pairs[it.x];
+  double *pos = a.priv + (size_t)(blockIdx.x % FDM_COPIES) * 2 * a.nbins, *neg = pos + a.nbins;
+  const int ti = (int)((P.K_i + 63) / 64);
+  const int m0 = (it.y % ti) * 64, j0 = (it.y / ti) * 64;
+  __shared__ double Em[64], Ej[64];
+  if (threadIdx.x < 64) {
+    const int m = m0 + threadIdx.x;
+    Em[threadIdx.x] = m < P.K_i ? P.E_i[m] : 0.0;
+  } else if (threadIdx.x < 128) {
+    const int j = j0 + threadIdx.x - 64;
+    Ej[threadIdx.x - 64] = j < P.K_j ? P.E_j[j] : 0.0;
+  }
+  __syncthreads();
+  const int ml = threadIdx.x & 63;
+  const int m = m0 + ml;
+  if (m >= P.K_i) return;
+  for (int jl = threadIdx.x >> 6; jl < 64; jl += 4) {
+    const int j = j0 + jl;
+    if (j >= P.K_j) break;
+    const double de = Ej[jl] - Em[ml];
+    const double energy = fabs(de);
+    if (energy < a.emin || energy > a.emax) continue;   // outside the step's window: no contribution
+    const size_t e = (size_t)j * P.ld_op + m, q = (size_t)j * P.K_i + m;
+    const double wA = P.pa[q] * P.op1[e];
+    const double wB = P.pb[q] * P.op2[e];
+    const double w = (wA + a.msign * wB) * P.factor;
+    int64_t idx;
+    double rem;
+    bool low;
+    bin_pos(energy, a.log10emin, a.bpd, a.nbins, idx, rem, low);
+    bin_add(a.discard, de >= 0.0 ? pos : neg, energy, w, idx, rem, low);
+  }
+}
+
+int dmnrg_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin, double emax,
+              int sign, double *out_pos, double *out_neg, hipStream_t st) {
+  if (npairs < 0 || (npairs > 0 && !pairs) || !grid || grid->nbins < 2 || (sign != 1 && sign != -1) || !out_pos ||
+      !out_neg || !(grid->bins_per_decade > 0) || !(emin >= 0) || !(emax >= emin)) {
+    set_error("nrg_dmnrg_bin: invalid arguments");
+    return NRG_E_ARG;
+  }
+  if (grid->nbins > (1 << 20)) {
+    set_error("nrg_dmnrg_bin: more than 2^20 bins");
+    return NRG_E_ARG;
+  }
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair &q = pairs[p];
+    if (q.dim_i < 0 || q.dim_j < 0 || q.K_i < 0 || q.K_j < 0 || q.K_i > q.dim_i || q.K_j > q.dim_j ||
+        (q.K_i && q.K_j && (!q.op1 || !q.op2 || !q.E_i || !q.E_j || q.ld_op < q.dim_i))) {
+      set_error("nrg_dmnrg_bin: pair " + std::to_string(p) + " is inconsistent");
+      return NRG_E_ARG;
+    }
+    if (q.dim_i > INT32_MAX / 2 || q.dim_j > INT32_MAX / 2 || q.ld_op > INT32_MAX || q.n_rho_i > INT32_MAX ||
+        q.n_rho_j > INT32_MAX) {
+      set_error("nrg_dmnrg_bin: dimension too large");
+      return NRG_E_ARG;
+    }
+    if (q.K_i && q.K_j && (!q.rho_i || q.n_rho_i < q.K_i || !q.rho_j || q.n_rho_j < q.K_j)) {
+      set_error("nrg_dmnrg_bin: pair " + std::to_string(p) + " needs density matrices covering its kept states");
+      return NRG_E_ARG;
+    }
+  }
+  std::vector<int64_t> off(npairs, -1);
+  int64_t tot = 0;
+  for (int p = 0; p < npairs; p++)
+    if (pairs[p].K_i > 0 && pairs[p].K_j > 0) { off[p] = tot; tot += 2 * pairs[p].K_i * pairs[p].K_j; }
+  if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
+  if (int r = ctx->ws.ensure(sizeof(double) * (size_t)FDM_COPIES * 2 * (size_t)grid->nbins)) return r;
+  double *prod = ctx->ws2.as<double>();
+  std::vector<GemmDesc> gd;
+  std::vector<DmnrgPairDev> pd;
+  std::vector<int> items;
+  for (int p = 0; p < npairs; p++) {
+    if (off[p] < 0) continue;
+    const nrg_fdm_pair &q = pairs[p];
+    DmnrgPairDev d{};
+    d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
+    d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
+    double *pa = prod + off[p], *pb = pa + q.K_i * q.K_j;
+    d.pa = pa; d.pb = pb;
+    // row-major PA (K_j x K_i) = op2[:K_j, :K_i] rho_i[:K_i, :K_i]^T: column-major PA^T = rho_i op2s^T
+    GemmDesc g{};
+    g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 1;
+    g.B = q.op2; g.ldb = (int)q.ld_op; g.tB = 0;
+    g.C = pa; g.ldc = (int)q.K_i;
+    g.M = (int)q.K_i; g.N = (int)q.K_j; g.K = (int)q.K_i; g.alpha = 1.0; g.beta = 0.0;
+    gd.push_back(g);
+    // row-major PB (K_j x K_i) = rho_j[:K_j, :K_j] op1[:K_j, :K_i]: column-major PB^T = op1s^T rho_j^T
+    GemmDesc h{};
+    h.A = q.op1; h.lda = (int)q.ld_op; h.tA = 0;
+    h.B = q.rho_j; h.ldb = (int)q.n_rho_j; h.tB = 0;
+    h.C = pb; h.ldc = (int)q.K_i;
+    h.M = (int)q.K_i; h.N = (int)q.K_j; h.K = (int)q.K_j; h.alpha = 1.0; h.beta = 0.0;
+    gd.push_back(h);
+    const int ti = (int)((q.K_i + 63) / 64), tj = (int)((q.K_j + 63) / 64);
+    for (int t = 0; t < ti * tj; t++) {
+      items.push_back((int)pd.size());
+      items.push_back(t);
+    }
+    pd.push_back(d);
+  }
+  std::vector<int> gitems;
+  gemm_items(gd, 0, gd.size(), gitems);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t bG = sizeof(GemmDesc) * gd.size(), bPd = sizeof(DmnrgPairDev) * pd.size();
+  const size_t oG = 0, oPd = al(bG), oGI = oPd + al(bPd), oI = oGI + al(sizeof(int) * gitems.size());
+  const size_t total = oI + sizeof(int) * items.size() + 8;
+  std::vector<char> blob(total);
+  if (bG) memcpy(blob.data() + oG, gd.data(), bG);
+  if (bPd) memcpy(blob.data() + oPd, pd.data(), bPd);
+  if (!gitems.empty()) memcpy(blob.data() + oGI, gitems.data(), sizeof(int) * gitems.size());
+  if (!items.empty()) memcpy(blob.data() + oI, items.data(), sizeof(int) * items.size());
+  NRG_HIP(hipEventRecord(ctx->ev[0], st));
+  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
+  if (!gitems.empty())
+    gemm_launch(ctx, st, (const GemmDesc *)ctx->desc.as<char>(oG), ctx->desc.as<int>(oGI), gitems.data(), gitems.size() / 2);
+  NRG_HIP(hipEventRecord(ctx->ev[1], st));
+  DmnrgArgs da;
+  da.pairs = (const DmnrgPairDev *)ctx->desc.as<char>(oPd);
+  da.emin = emin; da.emax = emax; da.msign = -(double)sign;
+  da.log10emin = grid->log10emin; da.bpd = grid->bins_per_decade; da.discard = grid->discard_immediately;
+  da.nbins = grid->nbins;
+  da.priv = ctx->ws.as<double>();
+  if (!items.empty()) {
+    const size_t np = (size_t)FDM_COPIES * 2 * grid->nbins;
+    NRG_TIMED(ctx, st, "k_fdm_zero", hipLaunchKernelGGL(k_fdm_zero, dim3((unsigned)std::min<size_t>((np + 255) / 256, 4096)),
+                                                        dim3(256), 0, st, da.priv, np));
+    NRG_TIMED(ctx, st, "k_dmnrg_bin", hipLaunchKernelGGL(k_dmnrg_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st,
+                                                         da, (const int2 *)ctx->desc.as<char>(oI)));
+    NRG_TIMED(ctx, st, "k_fdm_reduce", hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((grid->nbins + 255) / 256), 2),
+                                                          dim3(256), 0, st, da.priv, (int64_t)grid->nbins, 2, out_pos,
+                                                          out_neg, out_pos, out_neg));
   }
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[2], st));

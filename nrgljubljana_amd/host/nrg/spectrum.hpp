@@ -5,7 +5,9 @@
 // (greater and lesser halves, c++/spectrum.hpp:12-31): every subspace pair of a step with both operator
 // blocks present is handed to one nrg_fdm_bin call; the step's bins are merged into the running
 // spectrum (SpectrumRealFreq::mergeCFS, c++/spectrum.hpp:73-76) on the host; continuous() trims the
-// bins (Bins::trim) and broadens them onto the mesh on the GPU (nrg_broaden).
+// bins (Bins::trim) and broadens them onto the mesh on the GPU (nrg_broaden). DMNRGSpectrum is the
+// same loop for Algo_DMNRG (c++/algo_DMNRG.hpp:30-57) through nrg_dmnrg_bin, merged with the N/N+2
+// window (SpectrumRealFreq::mergeNN2, c++/spectrum.hpp:77-127).
 #pragma once
 #include "dmnrg.hpp"
 
@@ -189,6 +191,123 @@ class FDMSpectrum {
   double T_;
   DBuf<double> acc_;
   SpectrumBins gt_, ls_;
+};
+
+// Energy window of the N/N+2 patching (c++/params.hpp:650-659, SpectrumRealFreq::mergeNN2half):
+// E0 = P.goodE, Efactor = P.getEfactor() (Lambda for one channel, sqrt(Lambda) otherwise or with NN1).
+struct NN2Window {
+  double E0 = 2.0, Efactor = 2.0, NNtanh = 0.0;
+  bool ZBW = false, NN2avg = false;
+  double emin(double scale) const { return ZBW ? 0.0 : scale * E0; }
+  double ex(double scale) const { return scale * E0 * Efactor; }
+  double emax(double scale) const {
+    return ZBW ? std::numeric_limits<double>::max() : scale * E0 * Efactor * Efactor;
+  }
+};
+
+// windowfunction (c++/spectrum.hpp:91-108); first/last: step.first() / step.last()
+inline double windowfunction(double E, double Emin, double Ex, double Emax, bool first, bool last, const NN2Window &w) {
+  if (E <= Ex && last) return 1.0;
+  if (E >= Ex && first) return 1.0;
+  if (w.ZBW) return 1.0;
+  if (E <= Emin || E >= Emax) return 0.0;
+  auto fnc = [&w](double x) {
+    if (w.NNtanh <= 0) return x;
+    const double f0 = std::tanh(w.NNtanh * (0.0 - 0.5)), fx = std::tanh(w.NNtanh * (x - 0.5)),
+                 f1 = std::tanh(w.NNtanh * (1.0 - 0.5));
+    return (fx - f0) / (f1 - f0);
+  };
+  if (E <= Ex) return fnc((E - Emin) / (Ex - Emin));
+  return 1.0 - fnc((E - Ex) / (Emax - Ex));
+}
+
+// mergeNN2half (c++/spectrum.hpp:115-127): a step's bins into the running half spectrum
+inline void mergeNN2half(const BinGrid &g, std::vector<double> &full, const std::vector<double> &cs, double scale,
+                         bool first, bool last, const NN2Window &w) {
+  NRG_ASSERT(full.size() == cs.size() && cs.size() == g.nbins);
+  const double Emin = w.emin(scale), Ex = w.ex(scale), Emax = w.emax(scale);
+  for (size_t i = 0; i < g.nbins; i++) {
+    const double energy = g.energy(i);
+    if (Emin < energy && energy < Emax && cs[i] != 0.0)
+      full[i] += (w.NN2avg ? 0.5 : 1.0) * cs[i] * windowfunction(energy, Emin, Ex, Emax, first, last, w);
+  }
+}
+
+// Algo_DMNRG (c++/algo_DMNRG.hpp:16-57) over all pairs of a step, spectrum merged with the N/N+2 window
+class DMNRGSpectrum {
+ public:
+  DMNRGSpectrum(Device &dev, const BinGrid &grid, int sign, const NN2Window &window)
+      : dev_(dev), grid_(grid), sign_(sign), win_(window), acc_(2 * grid.nbins), spec_(grid.nbins), step_(grid.nbins) {
+    NRG_ASSERT(sign == 1 || sign == -1);
+  }
+  // One step: abs_zero energies of `diag`, operator blocks op(Ij, Ii) (keys {Ij, Ii}), the DM-NRG
+  // density matrices rho of the step. scale = step.scale(); merge = step.N_for_merging();
+  // first/last = step.first() / step.last().
+  void calc(const DiagInfo &diag, const MatrixElements &op1, const MatrixElements &op2, const DensMatElements &rho,
+            double scale, bool merge, bool first, bool last, const FactorFnc &ff, const CheckFnc &cf = nullptr,
+            int spin = 0) {
+    std::map<Invar, size_t> eoff;
+    std::vector<double> E;
+    for (const auto &[I, eig] : diag) {
+      const auto e = eig.values.all_abs_zero();
+      eoff[I] = E.size();
+      E.insert(E.end(), e.begin(), e.end());
+    }
+    DBuf<double> dE(E.size() + 1);
+    if (!E.empty())
+      hip_check(hipMemcpy(dE.get(), E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    std::vector<nrg_fdm_pair> pairs;
+    for (const auto &[Ip, diagp] : diag)
+      for (const auto &[I1, diag1] : diag) {
+        const Twoinvar II{I1, Ip};
+        const auto a = op1.find(II), b = op2.find(II);
+        if (a == op1.end() || b == op2.end() || (cf && !cf(I1, Ip, spin))) continue;
+        const DMatrix &o1 = a->second, &o2 = b->second;
+        NRG_ASSERT(size1(o1) == size1(o2) && size2(o1) == size2(o2));
+        nrg_fdm_pair p{};
+        p.dim_i = (int64_t)size2(o1);
+        p.dim_j = (int64_t)size1(o1);
+        p.K_i = (int64_t)std::min(diagp.Krange().second, (size_t)p.dim_i);
+        p.K_j = (int64_t)std::min(diag1.Krange().second, (size_t)p.dim_j);
+        if (!p.K_i || !p.K_j) continue;
+        p.E_i = dE.get() + eoff[Ip];
+        p.E_j = dE.get() + eoff[I1];
+        p.op1 = o1.data();
+        p.op2 = o2.data();
+        p.ld_op = (int64_t)size2(o1);
+        const auto ri = rho.find(Ip), rj = rho.find(I1);
+        NRG_ASSERT(ri != rho.end() && rj != rho.end());
+        p.rho_i = ri->second.data(); p.n_rho_i = (int64_t)size1(ri->second);
+        p.rho_j = rj->second.data(); p.n_rho_j = (int64_t)size1(rj->second);
+        p.factor = ff(Ip, I1);
+        pairs.push_back(p);
+      }
+    acc_.zero();
+    const nrg_bin_grid g = grid_.c();
+    const size_t nb = grid_.nbins;
+    nrg_check(nrg_dmnrg_bin(dev_.ctx(), pairs.data(), (int)pairs.size(), &g, win_.emin(scale), win_.emax(scale), sign_,
+                            acc_.get(), acc_.get() + nb, dev_.stream()),
+              "nrg_dmnrg_bin");
+    std::vector<double> h(2 * nb);
+    hip_check(hipMemcpy(h.data(), acc_.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    step_.pos.assign(h.begin(), h.begin() + nb);
+    step_.neg.assign(h.begin() + nb, h.end());
+    if (merge) {   // Algo_DMNRG::end -> SpectrumRealFreq::mergeNN2
+      mergeNN2half(grid_, spec_.pos, step_.pos, scale, first, last, win_);
+      mergeNN2half(grid_, spec_.neg, step_.neg, scale, first, last, win_);
+    }
+  }
+  const SpectrumBins &spectrum() const { return spec_; }   // the merged spectrum so far
+  const SpectrumBins &last_step() const { return step_; }  // the last step's bins before merging
+  const BinGrid &grid() const { return grid_; }
+
+ private:
+  Device &dev_;
+  BinGrid grid_;
+  int sign_;
+  NN2Window win_;
+  DBuf<double> acc_;
+  SpectrumBins spec_, step_;
 };
 
 }  // namespace NRG

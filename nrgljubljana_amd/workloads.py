@@ -135,6 +135,32 @@ def fdm_elements(pairs):
     return sum(p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"] for p in pairs)
 
 
+def dmnrg_step(npairs=48, kmin=16, kmax=1000, emax=10.0, seed=SEED + 5):
+    """One DM-NRG spectral step's subspace pairs (Algo_DMNRG, kept x kept elements only): kept dims
+    log-uniform in [kmin, kmax] (no discarded states), ascending abs_zero energies in [0, emax),
+    op1 = op2 Gaussian, density matrices random symmetric positive with unit trace."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(kmin), np.log(kmax), size=(npairs, 2))
+    dims = np.clip(np.rint(np.exp(u)), kmin, kmax).astype(np.int64)
+    pairs = []
+    for ki, kj in dims:
+        ki, kj = int(ki), int(kj)
+        Ei, Ej = np.sort(rng.uniform(0, emax, ki)), np.sort(rng.uniform(0, emax, kj))
+        op = rng.standard_normal((kj, ki))
+        rhos = []
+        for k in (ki, kj):
+            a = rng.standard_normal((k, k))
+            m = a @ a.T
+            rhos.append(m / np.trace(m))
+        pairs.append(dict(dim_i=ki, dim_j=kj, K_i=ki, K_j=kj, E_i=Ei, E_j=Ej, op=op, rho_i=rhos[0], rho_j=rhos[1]))
+    return pairs
+
+
+def dmnrg_flops(pairs):
+    """The two density-matrix products of every pair: 2 K_i K_j (K_i + K_j)."""
+    return sum(2.0 * p["K_i"] * p["K_j"] * (p["K_i"] + p["K_j"]) for p in pairs)
+
+
 def dm_step(nsub=48, combs=16, kmin=16, kmax=1000, seed=SEED + 4):
     """One backward step of the density-matrix sweep (cdmI, c++/dmnrg.hpp:54-80): nsub subspaces of
     step N-1 with kept dimensions log-uniform in [kmin, kmax]; each has `combs` children at step N

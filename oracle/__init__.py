@@ -58,6 +58,14 @@ def lib():
         L.oracle_fdm_bin.restype = ctypes.c_int
         L.oracle_jacobi_ld.argtypes = [ctypes.c_int, dp, dp, dp]
         L.oracle_jacobi_ld.restype = ctypes.c_int
+        L.oracle_dmnrg_bin.argtypes = [ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, dp, dp]
+        L.oracle_dmnrg_bin.restype = ctypes.c_int
+        L.oracle_window.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int] * 3 + [ctypes.c_double]
+        L.oracle_window.restype = ctypes.c_double
+        L.oracle_merge_nn2.argtypes = [ctypes.POINTER(BinGrid), dp, dp] + [ctypes.c_double] * 3 + \
+            [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int]
+        L.oracle_merge_nn2.restype = None
         L.oracle_broaden.restype = ctypes.c_int
         L.oracle_br_new.restype = ctypes.c_double
         L.oracle_br_new.argtypes = [ctypes.c_double] * 4
@@ -129,6 +137,27 @@ def fdm_bin(pairs, grid, T, wnf, sign, nbins):
     lib().oracle_fdm_bin(arr, len(pairs), ctypes.byref(grid), float(T), float(wnf), int(sign),
                          _dp(out[0]), _dp(out[1]), _dp(out[2]), _dp(out[3]))
     return out
+
+
+def dmnrg_bin(pairs, grid, emin, emax, sign, nbins):
+    """Algo_DMNRG::calc for all pairs: returns (pos, neg) bins."""
+    from nrgljubljana_amd.core import FdmPair
+    out = [np.zeros(nbins) for _ in range(2)]
+    arr = (FdmPair * max(len(pairs), 1))(*pairs)
+    lib().oracle_dmnrg_bin(arr, len(pairs), ctypes.byref(grid), float(emin), float(emax), int(sign),
+                           _dp(out[0]), _dp(out[1]))
+    return out
+
+
+def window(E, Emin, Ex, Emax, first=False, last=False, zbw=False, nntanh=0.0):
+    return lib().oracle_window(E, Emin, Ex, Emax, int(first), int(last), int(zbw), float(nntanh))
+
+
+def merge_nn2(grid, full, cs, Emin, Ex, Emax, first=False, last=False, zbw=False, nntanh=0.0, nn2avg=False):
+    """mergeNN2half: full += window-weighted cs (in place)."""
+    assert full.dtype == np.float64 and cs.dtype == np.float64
+    lib().oracle_merge_nn2(ctypes.byref(grid), _dp(full), _dp(np.ascontiguousarray(cs)), Emin, Ex, Emax, int(first),
+                           int(last), int(zbw), float(nntanh), int(nn2avg))
 
 
 def broaden(pos_e, pos_w, neg_e, neg_w, mesh, alpha, omega0):

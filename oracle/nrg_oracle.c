@@ -18,6 +18,10 @@
  *   oracle_gemm3          c++/numerics_ublas.hpp:108-140 (product / transform / rotate,
  *                         same evaluation order as the reference)
  *   oracle_broaden        c++/spectrum.hpp:180-209, c++/spectral.hpp:39-71 (BR_NEW broadening)
+ *   oracle_dmnrg_bin      c++/algo_DMNRG.hpp:30-52 (weights lambda and the kept x kept loop, inner
+ *                         sums as plain loops), ChainBinning::add as for FDM
+ *   oracle_window,        c++/spectrum.hpp:91-127 (windowfunction and mergeNN2half: the N/N+2
+ *   oracle_merge_nn2      patching of a step's bins into the running spectrum)
  *   oracle_fdm_bin        c++/algo_FDM.hpp:30-60,84-114 (term loops, gt then ls as in
  *                         Algo_FDM::calc 136-142), c++/numerics.hpp:283-306 (prod_fit),
  *                         c++/spectrum.hpp:19-28 (ChainBinning::add),
@@ -266,6 +270,70 @@ int oracle_fdm_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *gr
 #undef OP2
   }
   return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * DM-NRG spectral weights (Algo_DMNRG::calc, c++/algo_DMNRG.hpp:30-52). Pair fields as for FDM,
+ * with E_i, E_j the abs_zero energies and rho_i, rho_j the DM-NRG density matrices.
+ * ------------------------------------------------------------------------- */
+int oracle_dmnrg_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin, double emax,
+                     int sign, double *out_pos, double *out_neg) {
+  chain_t cb = {grid, out_pos, out_neg};
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair *q = &pairs[p];
+    const int64_t ld = q->ld_op;
+    const double *op1 = q->op1, *op2 = q->op2;
+    /* Ip = subspace i (kets, index rm), I1 = subspace j (bras, index rj) */
+    for (int64_t rm = 0; rm < q->K_i; rm++)
+      for (int64_t rj = 0; rj < q->K_j; rj++) {
+        const double Em = q->E_i[rm], Ej = q->E_j[rj];
+        const double energy = Ej - Em;
+        double weightA = 0.0, weightB = 0.0;
+        if (!(fabs(energy) < emin || fabs(energy) > emax)) {
+          double sumA = 0.0;
+          for (int64_t ri = 0; ri < q->K_i; ri++) sumA += op2[rj * ld + ri] * q->rho_i[rm * q->n_rho_i + ri];
+          weightA = sumA * op1[rj * ld + rm];
+          double sumB = 0.0;
+          for (int64_t ri = 0; ri < q->K_j; ri++) sumB += op1[ri * ld + rm] * q->rho_j[rj * q->n_rho_j + ri];
+          weightB = sumB * op2[rj * ld + rm];
+        }
+        chain_add(&cb, energy, weightA + (-sign) * weightB, q->factor);
+      }
+  }
+  return 0;
+}
+
+/* windowfunction (c++/spectrum.hpp:91-108) for the linear window (NNtanh = 0) or the tanh one.
+ * first/last: step.first() / step.last(); zbw: P.ZBW(). */
+double oracle_window(double E, double Emin, double Ex, double Emax, int first, int last, int zbw, double NNtanh) {
+  if (E <= Ex && last) return 1.0;
+  if (E >= Ex && first) return 1.0;
+  if (zbw) return 1.0;
+  if (E <= Emin || E >= Emax) return 0.0;
+  double x, f;
+  if (Emin < E && E <= Ex) x = (E - Emin) / (Ex - Emin);
+  else x = (E - Ex) / (Emax - Ex);
+  if (NNtanh > 0) {
+    const double f0 = tanh(NNtanh * (0.0 - 0.5)), fx = tanh(NNtanh * (x - 0.5)), f1 = tanh(NNtanh * (1.0 - 0.5));
+    f = (fx - f0) / (f1 - f0);
+  } else {
+    f = x;
+  }
+  return (Emin < E && E <= Ex) ? f : 1.0 - f;
+}
+
+/* mergeNN2half (c++/spectrum.hpp:115-127) for one half: full[i] += factor * cs[i] * window(energy_i)
+ * over the bins with Emin < energy_i < Emax, energy_i the grid energy of bin i. */
+void oracle_merge_nn2(const nrg_bin_grid *g, double *full, const double *cs, double Emin, double Ex, double Emax,
+                      int first, int last, int zbw, double NNtanh, int nn2avg) {
+  for (int64_t i = 0; i < g->nbins; i++) {
+    const double energy = pow(10.0, g->log10emin + (double)i / g->bins_per_decade);
+    const double w = cs[i];
+    if (Emin < energy && energy < Emax && w != 0.0) {
+      const double factor = nn2avg ? 0.5 : 1.0;
+      full[i] += factor * w * oracle_window(energy, Emin, Ex, Emax, first, last, zbw, NNtanh);
+    }
+  }
 }
 
 /* ---------------------------------------------------------------------------

@@ -16,6 +16,10 @@ int oracle_syev_batched(int nblk, const int64_t *n, const int64_t *off_H, double
 int oracle_gemm3(const nrg_gemm3_term *terms, int nterms);
 int oracle_fdm_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf, int sign,
                    double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg);
+int oracle_dmnrg_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin, double emax, int sign,
+                     double *out_pos, double *out_neg);
+void oracle_merge_nn2(const nrg_bin_grid *g, double *full, const double *cs, double Emin, double Ex, double Emax, int first,
+                      int last, int zbw, double NNtanh, int nn2avg);
 int oracle_broaden(const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e, const double *neg_w,
                    int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0, double *out_pos,
                    double *out_neg);
@@ -455,6 +459,75 @@ TEST(Spectrum, fdm_step_matches_oracle) {
     EXPECT_TRUE(err <= 1e-13 * scale);
     std::printf("    step %zu: %zu pairs, max bin deviation %.2e of total |weight| %.3e\n", step, pairs.size(), err, scale);
   }
+}
+
+TEST(Spectrum, dmnrg_step_matches_oracle) {
+  auto &c = chain();
+  Device &dev = Device::default_device();
+  const auto mult = c.sym.multfnc();
+  const size_t last = c.Nmax - 1;
+  DiagInfo dl = c.diags[last];
+  dl.truncate_perform();
+  DensMatElements rho = init_rho(1.0 / c.T * std::pow(2.0, -(double)last / 2), dl, mult);
+  std::vector<const DiagInfo *> dptr;
+  for (auto &d : c.diags) dptr.push_back(&d);
+  const size_t step = last - 2;
+  calc_densitymatrix(rho, c.store, c.sym, dptr, step, c.Nmax, dev);   // DM-NRG rho of `step`
+  DiagInfo &diag = c.diags[step];
+  MatrixElements op;
+  std::map<Twoinvar, Matrix> op_h;
+  for (const auto &[I, e] : diag) {
+    const Invar Ij{I.qn[0] + 1};
+    if (!diag.count(Ij)) continue;
+    Matrix m = random_matrix(diag.at(Ij).getnrall(), e.getnrall());
+    op_h[{Ij, I}] = m;
+    op[{Ij, I}] = DMatrix(m);
+  }
+  const BinGrid grid = BinGrid::standard(-1, -1, 1.0, std::pow(2.0, -(double)last / 2), 100, 1e-16);
+  NN2Window win;
+  win.E0 = 1.5;
+  win.Efactor = 2.0;
+  win.NN2avg = true;
+  const double scale = std::pow(2.0, -(double)step / 2);
+  const auto ff = [](const Invar &Ii, const Invar &) { return 1.0 + 0.25 * Ii.qn[0]; };
+  DMNRGSpectrum spec(dev, grid, -1, win);
+  spec.calc(diag, op, op, rho, scale, true, false, false, ff);
+  // oracle on host copies of the same pairs, then the N/N+2 merge of both halves
+  std::vector<nrg_fdm_pair> pairs;
+  std::map<Invar, std::vector<double>> E;
+  std::map<Invar, Matrix> rho_h;
+  for (const auto &[I, e] : diag) E[I] = e.values.all_abs_zero();
+  for (auto &[I, m] : rho) rho_h[I] = m.host();
+  for (const auto &[II, m] : op_h) {
+    const auto &[Ij, Ii] = II;
+    nrg_fdm_pair p{};
+    p.dim_i = (int64_t)m.size2(); p.dim_j = (int64_t)m.size1();
+    p.K_i = (int64_t)diag.at(Ii).Krange().second; p.K_j = (int64_t)diag.at(Ij).Krange().second;
+    p.E_i = E[Ii].data(); p.E_j = E[Ij].data();
+    p.op1 = m.data(); p.op2 = m.data(); p.ld_op = (int64_t)m.size2();
+    p.rho_i = rho_h[Ii].data(); p.n_rho_i = (int64_t)rho_h[Ii].size1();
+    p.rho_j = rho_h[Ij].data(); p.n_rho_j = (int64_t)rho_h[Ij].size1();
+    p.factor = ff(Ii, Ij);
+    pairs.push_back(p);
+  }
+  const nrg_bin_grid g = grid.c();
+  std::vector<double> bp(grid.nbins), bn(grid.nbins), fp(grid.nbins), fn(grid.nbins);
+  EXPECT_EQ(oracle_dmnrg_bin(pairs.data(), (int)pairs.size(), &g, win.emin(scale), win.emax(scale), -1, bp.data(), bn.data()),
+            0);
+  oracle_merge_nn2(&g, fp.data(), bp.data(), win.emin(scale), win.ex(scale), win.emax(scale), 0, 0, 0, 0.0, 1);
+  oracle_merge_nn2(&g, fn.data(), bn.data(), win.emin(scale), win.ex(scale), win.emax(scale), 0, 0, 0, 0.0, 1);
+  double scale_b = 0, err_b = 0, scale_f = 0, err_f = 0;
+  for (size_t i = 0; i < grid.nbins; i++) {
+    scale_b += std::abs(bp[i]) + std::abs(bn[i]);
+    scale_f += std::abs(fp[i]) + std::abs(fn[i]);
+    err_b = std::max({err_b, std::abs(spec.last_step().pos[i] - bp[i]), std::abs(spec.last_step().neg[i] - bn[i])});
+    err_f = std::max({err_f, std::abs(spec.spectrum().pos[i] - fp[i]), std::abs(spec.spectrum().neg[i] - fn[i])});
+  }
+  EXPECT_TRUE(scale_b > 0 && scale_f > 0);
+  EXPECT_TRUE(err_b <= 1e-13 * scale_b);
+  EXPECT_TRUE(err_f <= 1e-13 * scale_f);
+  std::printf("    step %zu: %zu pairs, max bin deviation %.2e of %.3e (merged %.2e of %.3e)\n", step, pairs.size(), err_b,
+              scale_b, err_f, scale_f);
 }
 
 TEST(Spectrum, continuous_matches_oracle) {
