@@ -535,6 +535,108 @@ def bench_dmnrg(args):
     print(json.dumps(out), flush=True)
 
 
+def ham_terms(sizes, blocks, ptr):
+    """nrg_ham_term list of a ham_step, in hamiltonian()'s order: energies, then per block the table's
+    offdiag calls and the zeta shifts (c++/core.hpp:72-88, c++/matrix.hpp:32-111)."""
+    from nrgljubljana_amd import HamTerm, NRG_HAM_BLOCK, NRG_HAM_DIAG
+    scale, sf, zeta = 0.37, 2.0 ** 0.5, 0.05
+    terms = []
+    for b, blk in enumerate(blocks):
+        dims = blk["dims"]
+        offs = np.concatenate([[0], np.cumsum(dims)[:-1]]).astype(int)
+        for i, d in enumerate(dims):
+            if d:
+                terms.append(HamTerm(NRG_HAM_DIAG, 0, b, int(offs[i]), int(offs[i]), d, d, ptr(blk["corr"][i]), 0, sf, 0.0))
+        for (i, j), m in blk["mats"].items():
+            if i > j:
+                terms.append(HamTerm(NRG_HAM_BLOCK, 1, b, int(offs[j - 1]), int(offs[i - 1]), dims[j - 1], dims[i - 1],
+                                     ptr(m), dims[j - 1], 0.6 / scale, 0.0))
+            else:
+                terms.append(HamTerm(NRG_HAM_BLOCK, 0, b, int(offs[i - 1]), int(offs[j - 1]), dims[i - 1], dims[j - 1],
+                                     ptr(m), dims[j - 1], 0.6 / scale, 0.0))
+        for i, d in enumerate(dims):
+            if d:
+                terms.append(HamTerm(NRG_HAM_DIAG, 0, b, int(offs[i]), int(offs[i]), d, d, None, 0, 0.0,
+                                     zeta * (i - 0.5) / scale))
+    return terms
+
+
+def bench_ham(args):
+    """Assembly of one step's Hamiltonian blocks in the eigensolver's arena (nrg_hamiltonian); value =
+    GB/s of algorithmic traffic (every arena element written once, every source element read once)."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, workloads
+    sizes, blocks = workloads.ham_step(nblk=args.nblk)
+    off = np.concatenate([[0], np.cumsum(sizes.astype(np.int64) ** 2)[:-1]]).astype(np.int64)
+    tot = int(np.sum(sizes.astype(np.int64) ** 2))
+    dev = torch.device("cuda:0")
+    keep = {}
+
+    def dptr(a):
+        if id(a) not in keep:
+            keep[id(a)] = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        return keep[id(a)].data_ptr()
+    from nrgljubljana_amd import HamTerm
+    terms = ham_terms(sizes, blocks, dptr)
+    terms_arr = (HamTerm * len(terms))(*terms)   # marshalled once, as a caller keeping its plan would
+    src_elems = sum(m.size for b in blocks for m in b["mats"].values()) + sum(int(np.sum(b["dims"])) for b in blocks)
+    abytes = 8.0 * (tot + src_elems)
+    H = torch.empty(tot, dtype=torch.float64, device=dev)
+    ctx = Context(0)
+
+    def step():
+        ctx.hamiltonian(sizes, off, H, terms_arr)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    kms = kt["k_ham_tile"][0] / 2
+    ach = abytes / (kms * 1e-3) / 1e9
+    # what the kernel replaces: building on the host and copying the arena over PCIe (pinned)
+    hbuf = torch.empty(tot, dtype=torch.float64).pin_memory()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        H.copy_(hbuf, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d_ms = (time.perf_counter() - t0) / 3 * 1e3
+    log(f"hamiltonian: {len(sizes)} blocks, {tot * 8 / 1e6:.1f} MB arena, {len(terms)} terms, {ms:.3f} ms per call "
+        f"(k_ham_tile {kms:.3f} ms); the same arena over PCIe from pinned memory: {h2d_ms:.3f} ms")
+    roof = {"bound": "hbm", "kernel": "k_ham_tile", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "note": "algorithmic bytes = 8 B per arena element written + 8 B per source element read"}
+    cpu = None
+    if not args.no_cpu:
+        hk = {}
+
+        def hptr(a):
+            hk[id(a)] = np.ascontiguousarray(a)
+            return hk[id(a)].ctypes.data
+        hterms = ham_terms(sizes, blocks, hptr)
+        Hh = np.empty(tot)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 1000:
+            oracle.hamiltonian(sizes, off, Hh, hterms)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        cpu = {"value": round(abytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"the whole step {reps} times (zero fill + term loops, hamiltonian()'s order), "
+                         f"{dt * 1e3:.1f} ms each"}
+    out = {"metric": "Hamiltonian assembly of one NRG step, algorithmic GB/s", "value": round(abytes / (ms * 1e-3) / 1e9, 2),
+           "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{len(sizes)} blocks, orders log-uniform [16,1024] (seed 20261203), 4 sub-blocks "
+                                  "each, 4 couplings (one transposed), energies and zeta shifts, seed 20261209",
+                      "arena_bytes": tot * 8, "terms": len(terms)},
+           "pcie_h2d_ms_same_arena": round(h2d_ms, 3), "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_broaden(args):
     """Broadening of one binned spectrum (1000 bins/decade over 14 decades, trimmed) onto the standard
     460-point mesh (nrg_broaden); value = BR_NEW evaluations per second."""
@@ -684,7 +786,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg"], default="eig")
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "ham"], default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--npairs", type=int, default=48)
@@ -696,7 +798,7 @@ def main():
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
-     "dm": bench_dm, "dmnrg": bench_dmnrg}[args.workload](args)
+     "dm": bench_dm, "dmnrg": bench_dmnrg, "ham": bench_ham}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -51,6 +51,45 @@ const char *nrg_last_error(void);
 const char *nrg_version(void);
 
 /*
+ * Hamiltonian assembly of all blocks of one NRG step into the eigensolver's input arena: replaces
+ * hamiltonian() (c++/core.hpp:72-88) with the symmetry-independent building steps it calls,
+ * Symmetry::offdiag_function_impl, diag_function_impl and diag_offdiag_function
+ * (c++/matrix.hpp:32-111). The symmetry's make_matrix table (which sub-blocks, which <||f||>
+ * matrix, which coefficient) is evaluated by the caller and handed over as a list of terms, in
+ * the order the reference applies them; each output element receives its terms in that order.
+ *
+ *   NRG_HAM_BLOCK  H[row .. row+rows, col .. col+cols] += factor * S        (trans = 0)
+ *                  H[row .. row+rows, col .. col+cols] += factor * S^T      (trans = 1)
+ *                  S: device, row-major, rows x cols (trans = 0) or cols x rows (trans = 1), leading
+ *                  dimension ld. offdiag_function: factor = coefficient / step.scale(); for i > j
+ *                  the reference adds the transpose into the upper block (trans = 1).
+ *   NRG_HAM_DIAG   H[row+k, row+k] += factor * src[k] + shift,  k < rows   (src may be NULL)
+ *                  the energies of hamiltonian(): factor = P.nrg_step_scale_factor(),
+ *                  src = the ancestor's corr energies; diag_function: src = NULL,
+ *                  shift = zeta * (number - f * avgoccup) / step.scale().
+ *   NRG_HAM_IDENT  H[row+k, col+k] += factor,  k < rows                 (diag_offdiag_function)
+ *
+ * Block b of the arena is a row-major n[b] x n[b] matrix at element offset off_H[b]; every element
+ * is written (zero where no term applies), so the arena needs no clearing and can be handed to
+ * nrg_syev_batched as it is.
+ */
+#define NRG_HAM_BLOCK 0
+#define NRG_HAM_DIAG  1
+#define NRG_HAM_IDENT 2
+typedef struct nrg_ham_term {
+  int32_t kind;
+  int32_t trans;
+  int64_t block;
+  int64_t row, col;
+  int64_t rows, cols;
+  const double *src; int64_t ld;
+  double factor, shift;
+} nrg_ham_term;
+
+int nrg_hamiltonian(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H_arena,
+                    const nrg_ham_term *terms, int nterms, void *stream);
+
+/*
  * Batched real symmetric eigensolver: replaces the per-block loop
  *   diagonalisations_OpenMP  (c++/core.hpp:90-110)  ->  diagonalise (c++/diag.hpp:261-292)
  *   -> diagonalise_dsyev     (c++/diag.hpp:61-82, LAPACK dsyev with UPLO='L' on the row-major upper triangle).

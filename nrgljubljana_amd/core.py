@@ -18,8 +18,20 @@ NRG_GEMM_FP64 = 0
 NRG_GEMM_OZAKI = 1
 
 EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
-            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
+            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_hamiltonian", "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
             "nrg_kernel_time", "nrg_kernel_names"]
+
+
+NRG_HAM_BLOCK = 0
+NRG_HAM_DIAG = 1
+NRG_HAM_IDENT = 2
+
+
+class HamTerm(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("trans", ctypes.c_int32), ("block", ctypes.c_int64),
+                ("row", ctypes.c_int64), ("col", ctypes.c_int64), ("rows", ctypes.c_int64), ("cols", ctypes.c_int64),
+                ("src", ctypes.c_void_p), ("ld", ctypes.c_int64), ("factor", ctypes.c_double),
+                ("shift", ctypes.c_double)]
 
 
 class Gemm3Term(ctypes.Structure):
@@ -70,6 +82,9 @@ def lib():
     L.nrg_fdm_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
                               ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, vp]
     L.nrg_fdm_bin.restype = ctypes.c_int
+    L.nrg_hamiltonian.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), vp,
+                                  ctypes.POINTER(HamTerm), ctypes.c_int, vp]
+    L.nrg_hamiltonian.restype = ctypes.c_int
     L.nrg_dmnrg_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, vp, vp, vp]
     L.nrg_dmnrg_bin.restype = ctypes.c_int
@@ -161,6 +176,18 @@ class Context:
                               _stream_ptr(stream))
         if r != 0:
             raise NrgError(r, "nrg_fdm_bin")
+
+    def hamiltonian(self, sizes, off_H, H, terms, stream=None):
+        """nrg_hamiltonian: assemble every block of the arena H (device float64 tensor) from `terms` (a list
+        of HamTerm, or a ctypes array of them built once by the caller)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        off_H = np.ascontiguousarray(off_H, dtype=np.int64)
+        arr = terms if isinstance(terms, ctypes.Array) else (HamTerm * max(len(terms), 1))(*terms)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        r = lib().nrg_hamiltonian(self._h, len(sizes), sizes.ctypes.data_as(i64p), off_H.ctypes.data_as(i64p),
+                                  ctypes.c_void_p(H.data_ptr()), arr, len(terms), _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_hamiltonian")
 
     def dmnrg_bin(self, pairs, grid, emin, emax, sign, out_pos, out_neg, stream=None):
         """nrg_dmnrg_bin: pairs carry abs_zero energies and the DM-NRG density matrices."""

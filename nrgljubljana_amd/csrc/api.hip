@@ -232,6 +232,21 @@ int nrg_dmnrg_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg
   }
 }
 
+int nrg_hamiltonian(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H_arena,
+                    const nrg_ham_term *terms, int nterms, void *stream) {
+  if (!ctx) {
+    set_error("nrg_hamiltonian: NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return hamiltonian_run(ctx, nblk, n, off_H, H_arena, terms, nterms, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("nrg_hamiltonian: ") + e.what());
+    return NRG_E_HIP;
+  }
+}
+
 int nrg_broaden(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
                 const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
                 double *out_pos, double *out_neg, void *stream) {

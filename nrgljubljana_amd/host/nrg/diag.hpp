@@ -76,27 +76,21 @@ struct BatchResult {
   std::vector<int64_t> n, off;
   std::vector<std::vector<double>> val;
 };
-inline BatchResult syev_batch(Device &dev, const std::vector<const Matrix *> &mats, int mode) {
+// nrg_syev_batched on blocks already in a device arena (block b: n[b] x n[b] at off[b])
+inline BatchResult syev_arena(Device &dev, std::shared_ptr<DBuf<double>> arena, std::vector<int64_t> n,
+                              std::vector<int64_t> off, int mode) {
   BatchResult r;
-  const size_t nb = mats.size();
-  r.n.resize(nb);
-  r.off.resize(nb);
+  const size_t nb = n.size();
+  r.arena = std::move(arena);
+  r.n = std::move(n);
+  r.off = std::move(off);
   std::vector<int64_t> offw(nb);
-  int64_t tot = 0, totw = 0;
+  int64_t totw = 0;
   for (size_t b = 0; b < nb; b++) {
-    NRG_ASSERT(is_square(*mats[b]));
-    r.n[b] = (int64_t)mats[b]->size1();
-    r.off[b] = tot;
     offw[b] = totw;
-    tot += r.n[b] * r.n[b];
     totw += r.n[b];
   }
-  std::vector<double> host((size_t)tot);
-  for (size_t b = 0; b < nb; b++)
-    if (r.n[b]) std::memcpy(host.data() + r.off[b], mats[b]->data(), sizeof(double) * (size_t)(r.n[b] * r.n[b]));
-  r.arena = std::make_shared<DBuf<double>>((size_t)tot + 1);
   DBuf<double> w((size_t)totw + 1);
-  hip_check(hipMemcpy(r.arena->get(), host.data(), sizeof(double) * (size_t)tot, hipMemcpyHostToDevice), "hipMemcpy");
   std::vector<int> info(nb, 0);
   nrg_check(nrg_syev_batched(dev.ctx(), (int)nb, r.n.data(), r.off.data(), r.arena->get(), offw.data(), w.get(), mode,
                              info.data(), dev.stream()),
@@ -109,6 +103,24 @@ inline BatchResult syev_batch(Device &dev, const std::vector<const Matrix *> &ma
   r.val.resize(nb);
   for (size_t b = 0; b < nb; b++) r.val[b].assign(wh.begin() + offw[b], wh.begin() + offw[b] + r.n[b]);
   return r;
+}
+
+inline BatchResult syev_batch(Device &dev, const std::vector<const Matrix *> &mats, int mode) {
+  const size_t nb = mats.size();
+  std::vector<int64_t> n(nb), off(nb);
+  int64_t tot = 0;
+  for (size_t b = 0; b < nb; b++) {
+    NRG_ASSERT(is_square(*mats[b]));
+    n[b] = (int64_t)mats[b]->size1();
+    off[b] = tot;
+    tot += n[b] * n[b];
+  }
+  std::vector<double> host((size_t)tot);
+  for (size_t b = 0; b < nb; b++)
+    if (n[b]) std::memcpy(host.data() + off[b], mats[b]->data(), sizeof(double) * (size_t)(n[b] * n[b]));
+  auto arena = std::make_shared<DBuf<double>>((size_t)tot + 1);
+  hip_check(hipMemcpy(arena->get(), host.data(), sizeof(double) * (size_t)tot, hipMemcpyHostToDevice), "hipMemcpy");
+  return syev_arena(dev, std::move(arena), std::move(n), std::move(off), mode);
 }
 }  // namespace detail
 

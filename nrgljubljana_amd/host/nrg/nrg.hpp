@@ -4,6 +4,7 @@
 #include "device.hpp"
 #include "eigen.hpp"
 #include "diag.hpp"
+#include "hamiltonian.hpp"
 #include "numerics.hpp"
 #include "recalc.hpp"
 #include "measurements.hpp"

@@ -135,6 +135,30 @@ def fdm_elements(pairs):
     return sum(p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"] for p in pairs)
 
 
+def ham_step(nblk=256, seed=SEED + 6):
+    """One step's Hamiltonian blocks with the eigensolver workload's orders (log-uniform [16,1024]):
+    each block split into 4 sub-blocks (the combs of a one-channel QS step), the ancestors' energies
+    on the diagonal, <||f||> couplings between sub-blocks (1,2), (1,3), (2,4), (3,4) as in a
+    one-channel table (the pair (3,4) given as (4,3), i.e. transposed), and a zeta shift per
+    sub-block. Returns (sizes, blocks), blocks[b] = dict(dims, corr, mats)."""
+    rng = np.random.default_rng(seed)
+    sizes = eig_sizes(nblk, seed=SEED)
+    blocks = []
+    for n in sizes:
+        n = int(n)
+        cuts = np.sort(rng.choice(np.arange(1, n), 3, replace=False)) if n > 3 else np.array([1, 2, 3])[:max(n - 1, 0)]
+        dims = np.diff(np.concatenate([[0], cuts, [n]])).astype(int).tolist()
+        while len(dims) < 4:
+            dims.append(0)
+        corr = [np.sort(rng.uniform(0, 10, d)) for d in dims]
+        mats = {}
+        for (i, j) in ((1, 2), (1, 3), (2, 4), (4, 3)):
+            if dims[i - 1] and dims[j - 1]:
+                mats[(i, j)] = rng.standard_normal((dims[i - 1], dims[j - 1]))
+        blocks.append(dict(dims=dims, corr=corr, mats=mats))
+    return sizes, blocks
+
+
 def dmnrg_step(npairs=48, kmin=16, kmax=1000, emax=10.0, seed=SEED + 5):
     """One DM-NRG spectral step's subspace pairs (Algo_DMNRG, kept x kept elements only): kept dims
     log-uniform in [kmin, kmax] (no discarded states), ascending abs_zero energies in [0, emax),

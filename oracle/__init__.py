@@ -58,6 +58,9 @@ def lib():
         L.oracle_fdm_bin.restype = ctypes.c_int
         L.oracle_jacobi_ld.argtypes = [ctypes.c_int, dp, dp, dp]
         L.oracle_jacobi_ld.restype = ctypes.c_int
+        from nrgljubljana_amd.core import HamTerm
+        L.oracle_hamiltonian.argtypes = [ctypes.c_int, i64p, i64p, dp, ctypes.POINTER(HamTerm), ctypes.c_int]
+        L.oracle_hamiltonian.restype = ctypes.c_int
         L.oracle_dmnrg_bin.argtypes = [ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, dp, dp]
         L.oracle_dmnrg_bin.restype = ctypes.c_int
@@ -137,6 +140,16 @@ def fdm_bin(pairs, grid, T, wnf, sign, nbins):
     lib().oracle_fdm_bin(arr, len(pairs), ctypes.byref(grid), float(T), float(wnf), int(sign),
                          _dp(out[0]), _dp(out[1]), _dp(out[2]), _dp(out[3]))
     return out
+
+
+def hamiltonian(sizes, off_H, arena, terms):
+    """hamiltonian() over all blocks: arena (float64, C-contiguous) is overwritten."""
+    from nrgljubljana_amd.core import HamTerm
+    sizes = np.ascontiguousarray(sizes, np.int64)
+    off_H = np.ascontiguousarray(off_H, np.int64)
+    assert arena.dtype == np.float64 and arena.flags.c_contiguous
+    arr = (HamTerm * max(len(terms), 1))(*terms)
+    lib().oracle_hamiltonian(len(sizes), _ip(sizes), _ip(off_H), _dp(arena), arr, len(terms))
 
 
 def dmnrg_bin(pairs, grid, emin, emax, sign, nbins):

@@ -20,6 +20,9 @@
  *   oracle_broaden        c++/spectrum.hpp:180-209, c++/spectral.hpp:39-71 (BR_NEW broadening)
  *   oracle_dmnrg_bin      c++/algo_DMNRG.hpp:30-52 (weights lambda and the kept x kept loop, inner
  *                         sums as plain loops), ChainBinning::add as for FDM
+ *   oracle_hamiltonian    c++/core.hpp:72-88 (zero matrix, then the terms in order),
+ *                         c++/matrix.hpp:32-111 (offdiag_function_impl, diag_function_impl,
+ *                         diag_offdiag_function)
  *   oracle_window,        c++/spectrum.hpp:91-127 (windowfunction and mergeNN2half: the N/N+2
  *   oracle_merge_nn2      patching of a step's bins into the running spectrum)
  *   oracle_fdm_bin        c++/algo_FDM.hpp:30-60,84-114 (term loops, gt then ls as in
@@ -299,6 +302,44 @@ int oracle_dmnrg_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *
         }
         chain_add(&cb, energy, weightA + (-sign) * weightB, q->factor);
       }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Hamiltonian assembly: hamiltonian() (c++/core.hpp:72-88) starts from a zero matrix and applies,
+ * in order, the energies on the diagonal and the symmetry table's offdiag/diag/diag_offdiag calls
+ * (c++/matrix.hpp:32-111); each call is one nrg_ham_term here.
+ * ------------------------------------------------------------------------- */
+int oracle_hamiltonian(int nblk, const int64_t *n, const int64_t *off_H, double *H, const nrg_ham_term *terms,
+                       int nterms) {
+  for (int b = 0; b < nblk; b++) memset(H + off_H[b], 0, sizeof(double) * (size_t)(n[b] * n[b]));
+  for (int k = 0; k < nterms; k++) {
+    const nrg_ham_term *t = &terms[k];
+    const int64_t N = n[t->block];
+    double *h = H + off_H[t->block];
+    if (t->kind == NRG_HAM_BLOCK) {
+      /* hsub += factor * mat, or factor * herm(mat) for i > j */
+      for (int64_t r = 0; r < t->rows; r++)
+        for (int64_t c = 0; c < t->cols; c++) {
+          const double m = t->trans ? t->src[c * t->ld + r] : t->src[r * t->ld + c];
+          const double v = t->factor * m;
+          h[(t->row + r) * N + (t->col + c)] += v;
+        }
+    } else if (t->kind == NRG_HAM_DIAG) {
+      /* h(r,r) = scale * corr(n) onto the zero matrix; h(j,j) += zeta (number - f avgoccup) / scale */
+      for (int64_t r = 0; r < t->rows; r++) {
+        double *x = &h[(t->row + r) * N + (t->row + r)];
+        if (t->src) {
+          const double v = t->factor * t->src[r];
+          *x += v;
+        }
+        *x += t->shift;
+      }
+    } else {
+      /* h(begin1 + l, begin2 + l) += factor_scaled */
+      for (int64_t l = 0; l < t->rows; l++) h[(t->row + l) * N + (t->col + l)] += t->factor;
+    }
   }
   return 0;
 }

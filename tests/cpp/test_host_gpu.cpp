@@ -160,6 +160,71 @@ TEST(Diag, diagonalisations_match_oracle) {
   }
 }
 
+TEST(Hamiltonian, assembled_blocks_have_the_closed_form_spectra) {
+  Device &dev = Device::default_device();
+  const double step_scale = 0.37, sf = std::sqrt(2.0);
+  StepHamiltonians hs;
+  // block 0: two sub-blocks of 150 states, energies a and b, coupled by t times the identity
+  // (diag_offdiag_function): eigenvalues (a+b)/2 -+ sqrt(((a-b)/2)^2 + t^2), 150 times each
+  const size_t m = 150;
+  const double c1 = 0.2, c2 = 0.9, t = 0.35;
+  const size_t b0 = hs.add_block(Invar{0}, SubspaceDimensions({m, m}, {Invar{0}, Invar{1}}));
+  hs.energies(b0, 0, std::vector<double>(m, c1), sf);
+  hs.energies(b0, 1, std::vector<double>(m, c2), sf);
+  hs.diag_offdiag(b0, 1, 2, t * step_scale, step_scale);
+  hs.diag_offdiag(b0, 2, 1, 5.0, step_scale);   // i > j: ignored, as in the reference
+  // block 1: sub-blocks of 90 and 130 states coupled by a term given for (2, 1): a 130 x 90 matrix
+  // F added transposed into the upper block, times f / step.scale().
+  // Eigenvalues: -+ the singular values of (f / scale) F, and 40 zeros
+  const size_t r1 = 90, r2 = 130;
+  const double f = -0.8;
+  Matrix F = random_matrix(r2, r1);
+  DMatrix Fd(F);
+  const size_t b1 = hs.add_block(Invar{1}, SubspaceDimensions({r1, r2}, {Invar{0}, Invar{1}}));
+  hs.offdiag(b1, 2, 1, Fd, f, step_scale);
+  EXPECT_EQ(hs.order(b1), r1 + r2);
+  const DiagParams DP;
+  DiagInfo diag = diagonalisations(hs, DP, dev);
+  {
+    const auto &v = diag.at(Invar{0}).values;
+    const double a = sf * c1, b = sf * c2, r = std::sqrt((a - b) / 2 * (a - b) / 2 + t * t);
+    double err = 0;
+    for (size_t k = 0; k < 2 * m; k++) err = std::max(err, std::abs(v.rel(k) - ((a + b) / 2 + (k < m ? -r : r))));
+    EXPECT_TRUE(err <= 1e-14);
+  }
+  {
+    // singular values of F from the oracle's dsyev of F^T F (90 x 90)
+    Matrix G(r1, r1);
+    for (size_t i = 0; i < r1; i++)
+      for (size_t j = 0; j < r1; j++) {
+        double s = 0;
+        for (size_t k = 0; k < r2; k++) s += F(k, i) * F(k, j);
+        G(i, j) = j >= i ? s : 0.0;
+      }
+    std::vector<double> w(r1);
+    std::vector<int64_t> n{(int64_t)r1}, off{0};
+    std::vector<int> info(1);
+    EXPECT_EQ(oracle_syev_batched(1, n.data(), off.data(), G.data(), off.data(), w.data(), 1, info.data()), 0);
+    std::vector<double> want;
+    const double g = std::abs(f / step_scale);
+    for (double x : w) {
+      const double sv = g * std::sqrt(std::max(x, 0.0));
+      want.push_back(sv);
+      want.push_back(-sv);
+    }
+    for (size_t k = 0; k < r2 - r1; k++) want.push_back(0.0);
+    std::sort(want.begin(), want.end());
+    const auto &v = diag.at(Invar{1}).values;
+    double err = 0, big = 0;
+    for (size_t k = 0; k < want.size(); k++) {
+      err = std::max(err, std::abs(v.rel(k) - want[k]));
+      big = std::max(big, std::abs(want[k]));
+    }
+    EXPECT_TRUE(err <= 1e-13 * big);
+    std::printf("    identity-coupled block and transposed-coupling block: max deviation %.2e of %.2e\n", err, big);
+  }
+}
+
 TEST(Numerics, product_transform_rotate_match_oracle) {
   const size_t m = 70, n = 90, k = 130, l = 65;
   {   // product M += f A B^T
