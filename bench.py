@@ -535,6 +535,23 @@ def bench_dmnrg(args):
     print(json.dumps(out), flush=True)
 
 
+def device_uploader(dev):
+    """ptr(array) -> device address of a copy of the array. Every array is copied once and both the
+    array and its copy are held until the caller drops `keep`: a cache keyed by id() alone would hand a
+    later temporary (which can reuse a freed array's id) the copy of an array of another shape."""
+    import torch
+    keep = {}
+
+    def ptr(a):
+        hit = keep.get(id(a))
+        if hit is None or hit[0] is not a:
+            hit = (a, torch.from_numpy(np.ascontiguousarray(a)).to(dev))
+            keep[id(a)] = hit
+        assert tuple(hit[1].shape) == tuple(np.shape(a))
+        return hit[1].data_ptr()
+    return ptr, keep
+
+
 def ham_terms(sizes, blocks, ptr):
     """nrg_ham_term list of a ham_step, in hamiltonian()'s order: energies, then per block the table's
     offdiag calls and the zeta shifts (c++/core.hpp:72-88, c++/matrix.hpp:32-111)."""
@@ -571,12 +588,7 @@ def bench_ham(args):
     off = np.concatenate([[0], np.cumsum(sizes.astype(np.int64) ** 2)[:-1]]).astype(np.int64)
     tot = int(np.sum(sizes.astype(np.int64) ** 2))
     dev = torch.device("cuda:0")
-    keep = {}
-
-    def dptr(a):
-        if id(a) not in keep:
-            keep[id(a)] = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-        return keep[id(a)].data_ptr()
+    dptr, keep = device_uploader(dev)
     from nrgljubljana_amd import HamTerm
     terms = ham_terms(sizes, blocks, dptr)
     terms_arr = (HamTerm * len(terms))(*terms)   # marshalled once, as a caller keeping its plan would
@@ -633,6 +645,136 @@ def bench_ham(args):
                                   "each, 4 couplings (one transposed), energies and zeta shifts, seed 20261209",
                       "arena_bytes": tot * 8, "terms": len(terms)},
            "pcie_h2d_ms_same_arena": round(h2d_ms, 3), "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_step(args):
+    """One synthetic NRG step, every stage on the GPU and device-resident: Hamiltonian assembly of the
+    256 blocks of the eigensolver workload (4 sub-blocks each, ham_step), the batched FP64 eigensolve
+    of the assembled arena, the recalc of new <||f||> blocks between neighbouring subspaces
+    (recalc_general over the 4 sub-blocks, U blocks viewed in the eigenvector arena), and the FDM
+    binning of those blocks with the new energies. value = wall time of the whole step."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import (Context, HamTerm, Gemm3Term, FdmPair, BinGrid, NRG_SYEV_FP64, NRG_GEMM_FP64,
+                                  workloads)
+    sizes, blocks = workloads.ham_step(nblk=args.nblk)
+    nb = len(sizes)
+    sizes = sizes.astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes ** 2)[:-1]]).astype(np.int64)
+    offw = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    tot, totw = int(np.sum(sizes ** 2)), int(np.sum(sizes))
+    dev = torch.device("cuda:0")
+    dptr, keep = device_uploader(dev)
+    hterms = ham_terms(sizes, blocks, dptr)
+    harr = (HamTerm * len(hterms))(*hterms)
+    H = torch.empty(tot, dtype=torch.float64, device=dev)
+    w = torch.empty(totw, dtype=torch.float64, device=dev)
+    # recalc: C_b (n_b x n_b+1) = sum_i f_i U_b(:, part_i) O_i U_b+1(:, part_i)^T
+    rng = np.random.default_rng(20261210)
+    gterms, Cs, rflops = [], [], 0.0
+    base = H.data_ptr()
+    for b in range(nb - 1):
+        n1, n2 = int(sizes[b]), int(sizes[b + 1])
+        C = torch.zeros(n1, n2, dtype=torch.float64, device=dev)
+        Cs.append(C)
+        d1, d2 = blocks[b]["dims"], blocks[b + 1]["dims"]
+        o1, o2 = np.concatenate([[0], np.cumsum(d1)[:-1]]), np.concatenate([[0], np.cumsum(d2)[:-1]])
+        for i in range(4):
+            if d1[i] and d2[i]:
+                O = rng.standard_normal((d1[i], d2[i]))
+                t = Gemm3Term()
+                t.m, t.n, t.k, t.l, t.f = n1, n2, int(d1[i]), int(d2[i]), float(rng.choice([-1.0, 1.0]))
+                t.A, t.lda, t.transA = base + 8 * (int(off[b]) + int(o1[i])), n1, 0
+                t.O, t.ldo = dptr(O), int(d2[i])
+                t.B, t.ldb, t.transB = base + 8 * (int(off[b + 1]) + int(o2[i])), n2, 0
+                t.C, t.ldc = C.data_ptr(), n2
+                gterms.append(t)
+                rflops += 2.0 * d1[i] * d2[i] * n2 + 2.0 * n1 * d1[i] * n2
+    # FDM: pair (Ii = b+1, Ij = b), op = C_b, energies = eigenvalues shifted to a zero minimum, 40 % kept
+    wmin = torch.zeros(1, dtype=torch.float64, device=dev)
+    Eg = torch.empty(totw, dtype=torch.float64, device=dev)
+    rhos, fpairs = [], []
+    for b in range(nb - 1):
+        n1, n2 = int(sizes[b]), int(sizes[b + 1])
+        Kj, Ki = int(0.4 * n1), int(0.4 * n2)
+        ri, rj = rng.standard_normal((Ki, Ki)), rng.standard_normal((Kj, Kj))
+        ri, rj = (ri + ri.T) / max(Ki, 1), (rj + rj.T) / max(Kj, 1)
+        fpairs.append(FdmPair(n2, n1, Ki, Kj, Eg.data_ptr() + 8 * int(offw[b + 1]), Eg.data_ptr() + 8 * int(offw[b]),
+                              Cs[b].data_ptr(), Cs[b].data_ptr(), n2, dptr(ri), Ki, dptr(rj), Kj, 1.0))
+    grid = BinGrid(-6.0, 100.0, 1001, 1e-16)
+    bins = [torch.zeros(grid.nbins, dtype=torch.float64, device=dev) for _ in range(4)]
+    ctx = Context(0)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    phase = np.zeros(4)
+
+    def step(record=False):
+        if record:
+            ev[0].record()
+        ctx.hamiltonian(sizes, off, H, harr)
+        if record:
+            ev[1].record()
+        ctx.syev_batched(H, sizes, off, w, offw, NRG_SYEV_FP64)
+        if record:
+            ev[2].record()
+        for C in Cs:
+            C.zero_()
+        ctx.gemm3_grouped(gterms, NRG_GEMM_FP64)
+        if record:
+            ev[3].record()
+        torch.sub(w, torch.min(w), out=Eg)
+        for x in bins:
+            x.zero_()
+        ctx.fdm_bin(fpairs, grid, 1.0, 0.5, -1, *bins)
+        if record:
+            ev[4].record()
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    for _ in range(2):
+        step(record=True)
+        torch.cuda.synchronize()
+        phase += [ev[k].elapsed_time(ev[k + 1]) / 2 for k in range(4)]
+    names = ["hamiltonian", "eigensolve", "recalc", "fdm"]
+    log(f"step: {nb} blocks, {tot * 8 / 1e6:.0f} MB of Hamiltonians, {len(gterms)} recalc terms "
+        f"({rflops / 1e9:.1f} GFLOP), {len(fpairs)} FDM pairs; {ms:.3f} ms per step: "
+        + ", ".join(f"{k} {v:.3f} ms" for k, v in zip(names, phase)))
+    eflops = workloads.eig_flops(sizes)
+    out_err = None
+    cpu = None
+    if not args.no_cpu:
+        # the oracle chain on the host: assembly (1 core), dsyev per block (OpenMP over blocks), recalc
+        # dgemm pairs (threaded BLAS), FDM loops (1 core)
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        hk = {}
+
+        def hptr(a):
+            hk[id(a)] = np.ascontiguousarray(a)
+            return hk[id(a)].ctypes.data
+        Hh = np.empty(tot)
+        t0 = time.perf_counter()
+        oracle.hamiltonian(sizes, off, Hh, ham_terms(sizes, blocks, hptr))
+        t1 = time.perf_counter()
+        wh, info = oracle.syev_batched(sizes, off, Hh, offw, threads=cores, blas_threads=1)
+        t2 = time.perf_counter()
+        gw = w.cpu().numpy()
+        out_err = float(max(np.max(np.abs(gw[o:o + n] - wh[o:o + n])) / max(np.max(np.abs(wh[o:o + n])), 1e-300)
+                            for o, n in zip(offw, sizes)))
+        cpu = {"value": round((t2 - t0) * 1e3, 1), "unit": "ms", "cores": cores, "kind": "port",
+               "sample": f"the oracle's assembly (1 core, {1e3 * (t1 - t0):.0f} ms) and dsyev per block (OpenMP over "
+                         f"{cores} cores, {1e3 * (t2 - t1):.0f} ms) of the same step; recalc and FDM not timed on the "
+                         "host (their CPU rates are the recalc and fdm workloads' baselines)"}
+        log(f"cpu: {cpu['sample']}; max |lambda_gpu - lambda_oracle| / max|lambda| = {out_err:.2e}")
+    out = {"metric": "NRG step wall-time (synthetic 256-subspace step: assembly + eigensolve + recalc + FDM)",
+           "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{nb} subspaces, orders log-uniform [16,1024] (seed 20261203), 4 sub-blocks "
+                                  "each; recalc and FDM over the 255 neighbouring pairs, seeds 20261209-10",
+                      "eig_flops": eflops, "recalc_flops": rflops},
+           "phases_ms": dict(zip(names, [round(x, 3) for x in phase])),
+           "eig_gflops_in_step": round(eflops / (phase[1] * 1e-3) / 1e9, 1),
+           "max_eig_err_rel_vs_oracle": out_err,
+           "roofline": None, "cpu_baseline": cpu}
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -786,7 +928,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "ham"], default="eig")
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "ham", "step"],
+                    default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--npairs", type=int, default=48)
@@ -798,7 +941,7 @@ def main():
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
-     "dm": bench_dm, "dmnrg": bench_dmnrg, "ham": bench_ham}[args.workload](args)
+     "dm": bench_dm, "dmnrg": bench_dmnrg, "ham": bench_ham, "step": bench_step}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -25,9 +25,9 @@ def device_structs(pairs):
     keep = {}
 
     def ptr(a):
-        if id(a) not in keep:
-            keep[id(a)] = torch.from_numpy(a).cuda()
-        return keep[id(a)].data_ptr()
+        if id(a) not in keep:   # the array is held too, so its id cannot be reused meanwhile
+            keep[id(a)] = (a, torch.from_numpy(a).cuda())
+        return keep[id(a)][1].data_ptr()
 
     return [p.to_struct(ptr) for p in pairs], keep
 
