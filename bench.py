@@ -675,9 +675,13 @@ def bench_step(args):
     rng = np.random.default_rng(20261210)
     gterms, Cs, rflops = [], [], 0.0
     base = H.data_ptr()
+    Carena = torch.zeros(int(sum(int(sizes[b]) * int(sizes[b + 1]) for b in range(nb - 1))), dtype=torch.float64,
+                         device=dev)   # the new operator blocks, one device arena
+    co = 0
     for b in range(nb - 1):
         n1, n2 = int(sizes[b]), int(sizes[b + 1])
-        C = torch.zeros(n1, n2, dtype=torch.float64, device=dev)
+        C = Carena[co:co + n1 * n2].view(n1, n2)
+        co += n1 * n2
         Cs.append(C)
         d1, d2 = blocks[b]["dims"], blocks[b + 1]["dims"]
         o1, o2 = np.concatenate([[0], np.cumsum(d1)[:-1]]), np.concatenate([[0], np.cumsum(d2)[:-1]])
@@ -718,8 +722,7 @@ def bench_step(args):
         ctx.syev_batched(H, sizes, off, w, offw, NRG_SYEV_FP64)
         if record:
             ev[2].record()
-        for C in Cs:
-            C.zero_()
+        Carena.zero_()
         ctx.gemm3_grouped(gterms, NRG_GEMM_FP64)
         if record:
             ev[3].record()
