@@ -10,6 +10,9 @@
 //                   the reflector and many workgroups per block form the symmetric product over
 //                   64x64 tiles of the lower triangle;
 //      k_gemm_f64   after each panel of NB columns, the trailing rank-2k update A22 -= V W^T + W V^T
+//                   (tiles on or below the first tile super-diagonal only)
+//      k_tail       the last trailing matrix of order <= TAIL (whole blocks up to TAIL), reduced
+//                   column by column in one workgroup's LDS, one launch for all blocks
 //   3. k_dc_prep    divide and conquer on the tridiagonal matrix (Cuppen's method with the
 //      k_dc_*       Gu-Eisenstat eigenvector formula, as LAPACK dstedc/dlaed0-3): leaves of order
 //      k_gemm_f64   1-2 at a common tree depth, then five merge launches per depth for all blocks
@@ -768,6 +771,9 @@ __device__ __forceinline__ int upper_bound_d(const double *v, int len, double x)
 __device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v >= hi ? hi - 1 : v); }
 
 constexpr int NT_DEF = 512;
+// dynamic LDS of k_dc_deflate for merges of order <= mc: d and z (doubles), predecessor (int) and
+// flags (byte) per index
+inline int dc_deflate_lds(int mc) { return (int)(2 * sizeof(double) * (size_t)mc + 5 * (size_t)mc + 64); }
 
 __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const int2 *items, int depth,
                                                        MergeState *mst) {
@@ -812,35 +818,86 @@ __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const
     mx = fmax(mx, fmax(fabs(dcat[i]), fabs(zcat[i])));
   }
   const double dzmax = block_max<NT_DEF>(mx, red);  // includes a barrier
+  const double EPS = 2.220446049250313e-16;
+  const double tol = 8.0 * EPS * dzmax;
+  // Parallel part of the deflation scan (dlaed2): which z are negligible, the previous
+  // non-negligible index of every index, and the Givens deflation test of every index against that
+  // predecessor with the unrotated values. The serial scan below then only repeats the test after a
+  // rotation has changed its predecessor, and computes the rotation itself when the test passes.
+  int *pv = (int *)(dyn + 2 * m);
+  unsigned char *flg = (unsigned char *)(pv + m);   // bit 0: negligible z, bit 1: the test deflates
+  __shared__ int carry[NT_DEF];
+  __shared__ int wlast[NT_DEF / WAVE];
+  double zm = 0;
+  for (int p = tid; p < m; p += NT_DEF) {
+    zm = fmax(zm, fabs(zsl[p]));
+    flg[p] = rho * fabs(zsl[p]) <= tol ? 1 : 0;
+  }
+  const double zmax = block_max<NT_DEF>(zm, red);
+  const int ch = (m + NT_DEF - 1) / NT_DEF, c0 = tid * ch, c1 = min(m, c0 + ch);
+  int last = -1;
+  for (int p = c0; p < c1; p++)
+    if (!(flg[p] & 1)) last = p;
+  {   // inclusive max-scan of `last` over the threads
+    const int lane = tid & (WAVE - 1), w = tid / WAVE;
+    int v = last;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int y = __shfl_up(v, o, WAVE);
+      if (lane >= o) v = max(v, y);
+    }
+    if (lane == WAVE - 1) wlast[w] = v;
+    __syncthreads();
+    for (int q = 0; q < w; q++) v = max(v, wlast[q]);
+    carry[tid] = v;
+    __syncthreads();
+  }
+  int cur = tid > 0 ? carry[tid - 1] : -1;
+  for (int p = c0; p < c1; p++) {
+    pv[p] = cur;
+    if (!(flg[p] & 1)) cur = p;
+  }
+  for (int p = c0; p < c1; p++) {
+    const int q = pv[p];
+    if ((flg[p] & 1) || q < 0) continue;
+    double s = zsl[q], c = zsl[p];
+    const double tau = hypot(c, s), t = dsl[p] - dsl[q];
+    c /= tau; s = -s / tau;
+    if (fabs(t * c * s) <= tol) flg[p] |= 2;
+  }
+  __syncthreads();
   if (tid == 0) {
-    const double EPS = 2.220446049250313e-16;
-    const double tol = 8.0 * EPS * dzmax;
-    double zmax = 0;
-    for (int p = 0; p < m; p++) zmax = fmax(zmax, fabs(zsl[p]));
     int k = 0, nd = 0, nrot = 0;
     if (rho * zmax <= tol) {
       for (int p = 0; p < m; p++) { cmp[p] = -1; cpos[m - 1 - nd] = p; nd++; }
     } else {
       int pj = -1;
+      bool rotated = false;   // pj's values were changed by the last rotation
       for (int nj = 0; nj < m; nj++) {
-        const double znj = zsl[nj];
-        if (rho * fabs(znj) <= tol) { cmp[nj] = -1; cpos[m - 1 - nd] = nj; nd++; continue; }
+        if (flg[nj] & 1) { cmp[nj] = -1; cpos[m - 1 - nd] = nj; nd++; continue; }
         if (pj < 0) { pj = nj; continue; }
-        double s = zsl[pj], c = znj;
-        const double tau = hypot(c, s), t = dsl[nj] - dsl[pj];
-        c /= tau; s = -s / tau;
-        if (fabs(t * c * s) <= tol) {
+        bool defl = (flg[nj] & 2) != 0;
+        double c = 0, s = 0, tau = 0;
+        if (rotated || defl) {
+          s = zsl[pj]; c = zsl[nj];
+          tau = hypot(c, s);
+          const double t = dsl[nj] - dsl[pj];
+          c /= tau; s = -s / tau;
+          defl = fabs(t * c * s) <= tol;
+        }
+        if (defl) {
           zsl[nj] = tau; zsl[pj] = 0.0;
           rpj[nrot] = pj; rnj[nrot] = nj; rc[nrot] = c; rs[nrot] = s; nrot++;
           const double dp = dsl[pj], dn = dsl[nj];
           dsl[pj] = dp * c * c + dn * s * s;
           dsl[nj] = dp * s * s + dn * c * c;
           cmp[pj] = -1; cpos[m - 1 - nd] = pj; nd++;
-          pj = nj;
+          rotated = true;
         } else {
           cmp[pj] = k; cpos[k] = pj; dnd[k] = dsl[pj]; znd[k] = zsl[pj]; k++;
-          pj = nj;
+          rotated = false;
         }
+        pj = nj;
       }
       if (pj >= 0) { cmp[pj] = k; cpos[k] = pj; dnd[k] = dsl[pj]; znd[k] = zsl[pj]; k++; }
     }
@@ -1863,7 +1920,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     int mc = 0;
     for (int depth = 0; depth < pl.Lmax; depth++) mc = std::max(mc, pl.mcap[depth]);
     const int lds3 = (int)(3 * sizeof(double) * (size_t)mc + 16), lds2 = (int)(2 * sizeof(double) * (size_t)mc + 16);
-    (void)hipFuncSetAttribute((const void *)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
+    (void)hipFuncSetAttribute((const void *)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize, dc_deflate_lds(mc));
     (void)hipFuncSetAttribute((const void *)k_dc_roots, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
     (void)hipFuncSetAttribute((const void *)k_dc_zhat, hipFuncAttributeMaxDynamicSharedMemorySize, lds3);
     (void)hipFuncSetAttribute((const void *)k_dc_normrank, hipFuncAttributeMaxDynamicSharedMemorySize, lds3);
@@ -1877,7 +1934,8 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     const int2 *mi = (const int2 *)(I + pl.mrg_off[depth]);
     const int2 *ch = (const int2 *)(I + pl.ch_off[depth]);
     const int2 *fc = (const int2 *)(I + pl.fc_off[depth]);
-    NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate, grid(pl.mrg_cnt[depth]), dim3(NT_DEF), lds2, st, aD,
+    NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate, grid(pl.mrg_cnt[depth]), dim3(NT_DEF),
+                                                          dc_deflate_lds(mc), st, aD,
                                                           mi, depth, L.mst));
     NRG_TIMED(ctx, st, "k_dc_roots", hipLaunchKernelGGL(k_dc_roots, grid(pl.rc_cnt[depth]), dim3(256), lds2, st, aD, mi,
                                                         (const int2 *)(I + pl.rc_off[depth]), depth, (const MergeState *)L.mst));
