@@ -77,7 +77,7 @@ def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs (profiles/), when
     they were measured on this workload; None otherwise."""
     path = os.path.join(ROOT, "profiles", "traffic_per_launch.json")
-    if args.mode != "fp64" or args.nblk != 256 or not os.path.exists(path):
+    if args.mode != "fp64" or args.nblk != 256 or (args.nmin, args.nmax) != (16, 1024) or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
@@ -116,7 +116,7 @@ def bench_eig(args):
         sys.exit("bench.py: the eigensolver forms are fp64 and mixed")
     mode = NRG_SYEV_FP64 if args.mode == "fp64" else NRG_SYEV_MIXED
     t0 = time.perf_counter()
-    sizes, off_H, arena, off_w, lams = workloads.eig_batch(nblk=args.nblk)
+    sizes, off_H, arena, off_w, lams = workloads.eig_batch(nblk=args.nblk, nmin=args.nmin, nmax=args.nmax)
     log(f"workload generated in {time.perf_counter() - t0:.1f} s: {len(sizes)} blocks, sum n = {int(sizes.sum())}, "
         f"max n = {int(sizes.max())}")
     flops = workloads.eig_flops(sizes)
@@ -222,7 +222,8 @@ def bench_eig(args):
         "vs_baseline": None,
         "dtype": "f64" if mode == NRG_SYEV_FP64 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "batched symmetric eigensolve, 256 blocks, n log-uniform in [16,1024], seed 20261203, "
+        "config": {"workload": f"batched symmetric eigensolve, {len(sizes)} blocks, n log-uniform in "
+                               f"[{args.nmin},{args.nmax}], seed 20261203, "
                                "clusters 1e-10 + degenerate pairs",
                    "form": args.mode, "blocks": int(len(sizes)), "sum_n": int(sizes.sum()), "max_n": int(sizes.max()),
                    "flops_per_step": flops, "alg_bytes_per_step": abytes},
@@ -935,6 +936,8 @@ def main():
                     default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
+    ap.add_argument("--nmin", type=int, default=16, help="eig: smallest block order")
+    ap.add_argument("--nmax", type=int, default=1024, help="eig: largest block order")
     ap.add_argument("--npairs", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
