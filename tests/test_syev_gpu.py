@@ -186,6 +186,6 @@ def test_step_batch_256_blocks(ctx, mode):
 @pytest.mark.parametrize("mode", MODES)
 def test_large_orders(ctx, mode):
     rng = np.random.default_rng(15)
-    # 1500 and 2048: four rows per thread in the column preparation; 3100: eight
+    # more than 1024 trailing rows: several rows per thread in the column preparation
     blocks = [workloads.eig_block(rng, n)[0] for n in [1500, 2048, 3100]]
     check_against_oracle(ctx, blocks, mode, label="large")
