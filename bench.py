@@ -455,6 +455,223 @@ def bench_fdm(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_cfs(args):
+    """CFS spectral weights + log binning of one synthetic step below the last (nrg_cfs_bin, Eqs. 15-16);
+    value = elements/s."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, FdmPair, BinGrid, workloads
+    pairs = workloads.fdm_step(npairs=args.npairs)
+    nel = workloads.cfs_elements(pairs)
+    flops = workloads.cfs_flops(pairs)
+    dev = torch.device("cuda:0")
+    grid = BinGrid(-12.0, 1000.0, 14001, 1e-16)
+    keep, structs = [], []
+    for p in pairs:
+        t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+        keep.append(t)
+        structs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], t["E_i"].data_ptr(), t["E_j"].data_ptr(),
+                               t["op"].data_ptr(), t["op"].data_ptr(), p["dim_i"], t["rho_i"].data_ptr(), p["K_i"],
+                               t["rho_j"].data_ptr(), p["K_j"], 1.0))
+    bins = [torch.zeros(grid.nbins, dtype=torch.float64, device=dev) for _ in range(4)]
+    ctx = Context(0)
+    T, Z = 1e-3, 1.0
+
+    def step():
+        for b in bins:
+            b.zero_()
+        ctx.cfs_bin(structs, grid, T, Z, False, -1, *bins)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log(f"cfs step: {len(pairs)} pairs, {nel / 1e6:.2f} M elements, {flops / 1e9:.2f} GFLOP of products, "
+        f"{ms:.3f} ms per call; " + ", ".join(f"{k} {v[0] / 2:.3f} ms ({v[1] // 2} launches)" for k, v in kt.items()))
+    gemm_ms = sum(v[0] for k, v in kt.items() if k.startswith("k_gemm")) / 2
+    bin_ms, bin_cnt = kt.get("k_cfs_bin", (0.0, 2))
+    if gemm_ms >= bin_ms / 2:
+        ach = flops / (gemm_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": " + ".join(sorted(k for k in kt if k.startswith("k_gemm"))),
+                "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None}
+    else:
+        abytes = 16.0 * nel + 8.0 * sum(p["dim_i"] + p["dim_j"] for p in pairs)
+        ach = abytes / (bin_ms / bin_cnt * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_cfs_bin", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "note": "algorithmic bytes = 16 B per element (an operator and a product) + 8 B per energy"}
+    cpu = None
+    if not args.no_cpu:
+        hs, hold = [], []
+        for p in pairs:
+            arrs = {k: np.ascontiguousarray(p[k]) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+            hold.append(arrs)
+            hs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], arrs["E_i"].ctypes.data,
+                              arrs["E_j"].ctypes.data, arrs["op"].ctypes.data, arrs["op"].ctypes.data, p["dim_i"],
+                              arrs["rho_i"].ctypes.data, p["K_i"], arrs["rho_j"].ctypes.data, p["K_j"], 1.0))
+        oracle.lib().oracle_blas_threads(1)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 500:
+            oracle.cfs_bin(hs, grid, T, Z, False, -1, grid.nbins)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * nel / dt / 1e6, 2), "unit": "Melements/s", "cores": 1, "kind": "port",
+               "sample": f"the whole step ({len(pairs)} pairs) {reps} times, dgemm products (1 thread) + scalar "
+                         f"term loops + serial binning, {dt:.1f} s"}
+    out = {"metric": "CFS spectral weights + log binning, elements per second", "value": round(nel / ms / 1e3, 2),
+           "unit": "Melements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"one CFS step below the last, {len(pairs)} subspace pairs, dims log-uniform "
+                                  "[16,2000], 40 % kept, 1000 bins/decade over 14 decades, seed 20261206",
+                      "elements": nel, "product_flops": flops},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_ft(args):
+    """FT spectral weights + log binning of one synthetic step (nrg_ft_bin, kept x kept); value = elements/s."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, FdmPair, BinGrid, workloads
+    pairs = workloads.dmnrg_step(npairs=args.npairs)
+    nel = sum(p["K_i"] * p["K_j"] for p in pairs)
+    dev = torch.device("cuda:0")
+    grid = BinGrid(-12.0, 1000.0, 14001, 1e-16)
+    T, Z = 0.5, 40.0
+    keep, structs = [], []
+    for p in pairs:
+        t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev) for k in ("E_i", "E_j", "op")}
+        keep.append(t)
+        structs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], t["E_i"].data_ptr(), t["E_j"].data_ptr(),
+                               t["op"].data_ptr(), t["op"].data_ptr(), p["dim_i"], None, 0, None, 0, 1.0))
+    bins = [torch.zeros(grid.nbins, dtype=torch.float64, device=dev) for _ in range(2)]
+    ctx = Context(0)
+
+    def step():
+        for b in bins:
+            b.zero_()
+        ctx.ft_bin(structs, grid, T, Z, -1, *bins)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log(f"ft step: {len(pairs)} pairs, {nel / 1e6:.2f} M elements, {ms:.3f} ms per call; "
+        + ", ".join(f"{k} {v[0] / 2:.3f} ms" for k, v in kt.items()))
+    tms, cnt = kt["k_cfs_bin"]
+    abytes = 16.0 * nel + 8.0 * sum(p["K_i"] + p["K_j"] for p in pairs)
+    ach = abytes / (tms / cnt * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_cfs_bin", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "note": "algorithmic bytes = 16 B per element (op1, op2) + 8 B per energy"}
+    cpu = None
+    if not args.no_cpu:
+        hs, hold, el = [], [], 0
+        for p in sorted(pairs, key=lambda q: q["K_i"] * q["K_j"])[:len(pairs) // 2]:
+            arrs = {k: np.ascontiguousarray(p[k]) for k in ("E_i", "E_j", "op")}
+            hold.append(arrs)
+            hs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], arrs["E_i"].ctypes.data,
+                              arrs["E_j"].ctypes.data, arrs["op"].ctypes.data, arrs["op"].ctypes.data, p["dim_i"],
+                              None, 0, None, 0, 1.0))
+            el += p["K_i"] * p["K_j"]
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 5000:
+            oracle.ft_bin(hs, grid, T, Z, -1, grid.nbins)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * el / dt / 1e6, 3), "unit": "Melements/s", "cores": 1, "kind": "port",
+               "sample": f"the {len(hs)} smallest pairs ({el / 1e6:.2f} M elements) {reps} times, the reference's "
+                         f"loop + serial binning, {dt:.1f} s"}
+    out = {"metric": "FT spectral weights + log binning, elements per second", "value": round(nel / ms / 1e3, 2),
+           "unit": "Melements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"one FT step, {len(pairs)} subspace pairs, kept dims log-uniform [16,1000], "
+                                  "1000 bins/decade over 14 decades, seed 20261208", "elements": nel},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_mats(args):
+    """Matsubara Green's function of one synthetic FDM step (nrg_fdm_mats, 100 frequencies, the
+    reference's default P.mats); value = (element, frequency) evaluations per second."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, FdmPair, workloads
+    pairs = workloads.fdm_step(npairs=args.npairs)
+    nel = workloads.fdm_elements(pairs)
+    mats = args.mats
+    evals = float(nel) * mats
+    dev = torch.device("cuda:0")
+    T, wnf = 0.05, 0.37
+    keep, structs = [], []
+    for p in pairs:
+        t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+        keep.append(t)
+        structs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], t["E_i"].data_ptr(), t["E_j"].data_ptr(),
+                               t["op"].data_ptr(), t["op"].data_ptr(), p["dim_i"], t["rho_i"].data_ptr(), p["K_i"],
+                               t["rho_j"].data_ptr(), p["K_j"], 1.0))
+    G = torch.zeros(2 * mats, dtype=torch.float64, device=dev)
+    ctx = Context(0)
+
+    def step():
+        G.zero_()
+        ctx.mats("fdm", structs, mats, T, -1, G, wnf=wnf)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log(f"mats step: {len(pairs)} pairs, {nel / 1e6:.2f} M elements x {mats} frequencies, {ms:.3f} ms per call; "
+        + ", ".join(f"{k} {v[0] / 2:.3f} ms" for k, v in kt.items()))
+    tms, cnt = kt["k_mats_fdm"]
+    # issued FP64 work per (element, frequency): E^2 + w^2, two Newton steps of the reciprocal (2 FMA
+    # each) and the two accumulating FMAs: 13 flops, plus one v_rcp_f64
+    ach = 13.0 * evals / (tms / cnt * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_mats_fdm", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "note": "FP64 vector work, 13 flops + 1 reciprocal per (element, frequency); peak = FP64 vector"}
+    cpu = None
+    if not args.no_cpu:
+        hs, hold, el = [], [], 0
+        for p in sorted(pairs, key=lambda q: q["dim_i"] * q["dim_j"])[:len(pairs) // 4]:
+            arrs = {k: np.ascontiguousarray(p[k]) for k in ("E_i", "E_j", "op", "rho_i", "rho_j")}
+            hold.append(arrs)
+            hs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], arrs["E_i"].ctypes.data,
+                              arrs["E_j"].ctypes.data, arrs["op"].ctypes.data, arrs["op"].ctypes.data, p["dim_i"],
+                              arrs["rho_i"].ctypes.data, p["K_i"], arrs["rho_j"].ctypes.data, p["K_j"], 1.0))
+            el += p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"]
+        oracle.lib().oracle_blas_threads(1)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 100:
+            oracle.mats("fdm", hs, mats, T, -1, wnf=wnf)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * el * mats / dt / 1e6, 2), "unit": "Mevaluations/s", "cores": 1, "kind": "port",
+               "sample": f"the {len(hs)} smallest pairs ({el / 1e6:.3f} M elements x {mats} frequencies) {reps} "
+                         f"times, the reference's term loops with C complex division, {dt:.1f} s"}
+    out = {"metric": "FDM Matsubara Green's function, (element, frequency) evaluations per second",
+           "value": round(evals / ms / 1e3, 2), "unit": "Mevaluations/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"one FDM step, {len(pairs)} subspace pairs, dims log-uniform [16,2000], 40 % "
+                                  f"kept, {mats} fermionic Matsubara frequencies, seed 20261206",
+                      "elements": nel, "frequencies": mats},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_dmnrg(args):
     """DM-NRG spectral weights + log binning of one synthetic step (nrg_dmnrg_bin); value = elements/s."""
     import torch
@@ -932,13 +1149,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "ham", "step"],
+    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "cfs", "ft", "mats", "ham", "step"],
                     default="eig")
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=16, help="eig: smallest block order")
     ap.add_argument("--nmax", type=int, default=1024, help="eig: largest block order")
     ap.add_argument("--npairs", type=int, default=48)
+    ap.add_argument("--mats", type=int, default=100, help="mats: Matsubara frequencies")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -947,7 +1165,7 @@ def main():
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
-     "dm": bench_dm, "dmnrg": bench_dmnrg, "ham": bench_ham, "step": bench_step}[args.workload](args)
+     "dm": bench_dm, "dmnrg": bench_dmnrg, "cfs": bench_cfs, "ft": bench_ft, "mats": bench_mats, "ham": bench_ham, "step": bench_step}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -185,6 +185,68 @@ int nrg_fdm_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_b
                 double *ls_neg, void *stream);
 
 /*
+ * CFS spectral weights of all subspace pairs of one step, binned: replaces the pair loop of
+ * BaseSpectrum::calc (c++/algo.hpp:55-67) driving Algo_CFS::calc = Algo_CFSgt + Algo_CFSls
+ * (c++/algo_CFS.hpp:13-144) and ChainBinning::add. Same pair descriptors as nrg_fdm_bin, with E_i,
+ * E_j the abs_zero energies and rho_i, rho_j the DM-NRG density matrices (Ii plays the reference's
+ * Ip, Ij its I1). At energy E_j - E_i, times the pair factor:
+ *   last_step != 0 (Eq. 11), kept i x kept j:  gt  op1 op2 exp(-E_i/T) / Z
+ *                                              ls  op1 op2 exp(-E_j/T) (-sign) / Z
+ *   otherwise (Eqs. 15-16):   discarded j x kept i:  ls  op1 (op2 rho_i) (-sign)
+ *                             kept j x discarded i:  gt  (rho_j^T op1) op2
+ * (products over the states the density matrices cover, as prod_fit_left / prod_adj_fit_left,
+ * c++/numerics.hpp:283-316). Z = stats.Zft. Accumulates (+=) into the four arrays, as nrg_fdm_bin.
+ */
+int nrg_cfs_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z,
+                int last_step, int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, void *stream);
+
+/*
+ * FT spectral weights of all subspace pairs of one step, binned: replaces the pair loop of
+ * BaseSpectrum::calc (c++/algo.hpp:55-67) driving Algo_FT::calc (c++/algo_FT.hpp:28-44) and
+ * ChainBinning::add. Pair descriptors as nrg_cfs_bin (no density matrices read). Every kept i x kept j
+ * element adds op1 op2 ((-sign) exp(-E_j/T) + exp(-E_i/T)) / Z at E_j - E_i, times the pair factor,
+ * into (out_pos, out_neg) (+=); the caller merges the step with the N/N+2 window (Algo_FT::end).
+ */
+int nrg_ft_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z,
+               int sign, double *out_pos, double *out_neg, void *stream);
+
+/*
+ * Matsubara-axis Green's functions of one step: replace the pair loop of BaseSpectrum::calc
+ * (c++/algo.hpp:55-67) driving Algo_FDMmats::calc (c++/algo_FDM.hpp:167-213),
+ * Algo_DMNRGmats::calc (c++/algo_DMNRG.hpp:74-100) or Algo_FTmats::calc (c++/algo_FT.hpp:59-84),
+ * and ChainMatsubara::add (c++/spectrum.hpp:34-42). Pair descriptors as nrg_fdm_bin (FDM: abs_G
+ * energies, rhoFDM) or nrg_dmnrg_bin (DM-NRG, FT: abs_zero energies, rho; FT reads no rho).
+ * G holds mats complex values, (Re, Im) interleaved as std::complex<double>; every call adds
+ * (+=) factor * term(n) of every evaluated element for n = 0 .. mats-1, at the frequencies
+ * ww(n) = T pi (2n+1) (sign = -1, fermions) or T pi 2n (sign = 1, bosons), c++/matsubara.hpp:20-27.
+ * mats <= 32767 (ww takes a short).
+ */
+int nrg_fdm_mats(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double wnf, int sign,
+                 double *G, void *stream);
+int nrg_dmnrg_mats(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, int sign, double *G,
+                   void *stream);
+/* Z = stats.Zft */
+int nrg_ft_mats(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double Z, int sign,
+                double *G, void *stream);
+
+/*
+ * Temperature-dependent sums of one step: replace Algo_GT<n>::calc (n = 0, 1, 2: GT, I1T, I2T,
+ * c++/algo_FT.hpp:95-134) and Algo_CHIT::calc (c++/algo_FT.hpp:136-170) over all pairs. Pair
+ * descriptors as nrg_ft_bin (abs_zero energies, no density matrices). With beta = 1 / temperature
+ * (temperature = P.gtp * step.scale() or P.chitp * step.scale()), over kept i x kept j:
+ *   NRG_TEMP_GT/I1T/I2T:  op1 op2 beta / (exp(beta E_j) + exp(beta E_i)) (E_j - E_i)^n / Z   (Z = stats.Zgt)
+ *   NRG_TEMP_CHIT:        op1 op2 chit_weight(E_j, E_i, beta) / Z                           (Z = stats.Zchit)
+ * times the pair factor, all added (+=) into *out (one double in device memory); the caller adds
+ * it at `temperature` (ChainTempDependence::add).
+ */
+#define NRG_TEMP_GT 0
+#define NRG_TEMP_I1T 1
+#define NRG_TEMP_I2T 2
+#define NRG_TEMP_CHIT 3
+int nrg_temp_sum(nrg_ctx *ctx, int kind, const nrg_fdm_pair *pairs, int npairs, double temperature, double Z,
+                 double *out, void *stream);
+
+/*
  * DM-NRG spectral weights of all subspace pairs of one step, binned: replaces the pair loop of
  * BaseSpectrum::calc (c++/algo.hpp:55-67) driving Algo_DMNRG::calc (c++/algo_DMNRG.hpp:30-52)
  * and ChainBinning::add (c++/spectrum.hpp:19-28). Same pair descriptors as nrg_fdm_bin, except that

@@ -18,10 +18,13 @@ NRG_GEMM_FP64 = 0
 NRG_GEMM_OZAKI = 1
 
 EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
-            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_hamiltonian", "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
+            "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_cfs_bin", "nrg_ft_bin", "nrg_fdm_mats", "nrg_dmnrg_mats",
+            "nrg_ft_mats", "nrg_temp_sum", "nrg_broaden", "nrg_hamiltonian",
+            "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
             "nrg_kernel_time", "nrg_kernel_names"]
 
 
+NRG_TEMP_GT, NRG_TEMP_I1T, NRG_TEMP_I2T, NRG_TEMP_CHIT = 0, 1, 2, 3
 NRG_HAM_BLOCK = 0
 NRG_HAM_DIAG = 1
 NRG_HAM_IDENT = 2
@@ -88,6 +91,24 @@ def lib():
     L.nrg_dmnrg_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, vp, vp, vp]
     L.nrg_dmnrg_bin.restype = ctypes.c_int
+    L.nrg_cfs_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
+                              ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp]
+    L.nrg_cfs_bin.restype = ctypes.c_int
+    L.nrg_ft_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int, vp, vp, vp]
+    L.nrg_ft_bin.restype = ctypes.c_int
+    L.nrg_fdm_mats.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_int, vp, vp]
+    L.nrg_fdm_mats.restype = ctypes.c_int
+    L.nrg_dmnrg_mats.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                                 ctypes.c_int, vp, vp]
+    L.nrg_dmnrg_mats.restype = ctypes.c_int
+    L.nrg_ft_mats.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_int, vp, vp]
+    L.nrg_ft_mats.restype = ctypes.c_int
+    L.nrg_temp_sum.argtypes = [vp, ctypes.c_int, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.c_double,
+                               ctypes.c_double, vp, vp]
+    L.nrg_temp_sum.restype = ctypes.c_int
     L.nrg_broaden.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_double,
                               ctypes.c_double, vp, vp, vp]
     L.nrg_broaden.restype = ctypes.c_int
@@ -197,6 +218,48 @@ class Context:
                                 _stream_ptr(stream))
         if r != 0:
             raise NrgError(r, "nrg_dmnrg_bin")
+
+    def cfs_bin(self, pairs, grid, T, Z, last, sign, gt_pos, gt_neg, ls_pos, ls_neg, stream=None):
+        """nrg_cfs_bin: pairs carry abs_zero energies and the DM-NRG density matrices; T and Z (stats.Zft)
+        enter at the last step only."""
+        arr = (FdmPair * max(len(pairs), 1))(*pairs)
+        r = lib().nrg_cfs_bin(self._h, arr, len(pairs), ctypes.byref(grid), float(T), float(Z), int(bool(last)),
+                              int(sign), _ptr(gt_pos), _ptr(gt_neg), _ptr(ls_pos), _ptr(ls_neg), _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_cfs_bin")
+
+    def ft_bin(self, pairs, grid, T, Z, sign, out_pos, out_neg, stream=None):
+        """nrg_ft_bin: kept x kept weights of Algo_FT (abs_zero energies, Z = stats.Zft)."""
+        arr = (FdmPair * max(len(pairs), 1))(*pairs)
+        r = lib().nrg_ft_bin(self._h, arr, len(pairs), ctypes.byref(grid), float(T), float(Z), int(sign),
+                             _ptr(out_pos), _ptr(out_neg), _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_ft_bin")
+
+    def mats(self, algo, pairs, mats, T, sign, G, wnf=1.0, Z=1.0, stream=None):
+        """nrg_{fdm,dmnrg,ft}_mats: G is a float64 device tensor of 2 * mats values ((Re, Im) pairs)."""
+        arr = (FdmPair * max(len(pairs), 1))(*pairs)
+        L = lib()
+        if algo == "fdm":
+            r = L.nrg_fdm_mats(self._h, arr, len(pairs), int(mats), float(T), float(wnf), int(sign), _ptr(G),
+                               _stream_ptr(stream))
+        elif algo == "dmnrg":
+            r = L.nrg_dmnrg_mats(self._h, arr, len(pairs), int(mats), float(T), int(sign), _ptr(G), _stream_ptr(stream))
+        elif algo == "ft":
+            r = L.nrg_ft_mats(self._h, arr, len(pairs), int(mats), float(T), float(Z), int(sign), _ptr(G),
+                              _stream_ptr(stream))
+        else:
+            raise ValueError(f"unknown Matsubara algorithm {algo!r}")
+        if r != 0:
+            raise NrgError(r, f"nrg_{algo}_mats")
+
+    def temp_sum(self, kind, pairs, temperature, Z, out, stream=None):
+        """nrg_temp_sum: adds the step's G(T)/I1(T)/I2(T)/chi(T) sum into out (a 1-element device tensor)."""
+        arr = (FdmPair * max(len(pairs), 1))(*pairs)
+        r = lib().nrg_temp_sum(self._h, int(kind), arr, len(pairs), float(temperature), float(Z), _ptr(out),
+                               _stream_ptr(stream))
+        if r != 0:
+            raise NrgError(r, "nrg_temp_sum")
 
     def broaden(self, pos_e, pos_w, neg_e, neg_w, mesh, alpha, omega0, out_pos, out_neg, stream=None):
         """nrg_broaden on device tensors (float64)."""

@@ -247,6 +247,81 @@ int nrg_hamiltonian(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off
   }
 }
 
+int nrg_cfs_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z,
+                int last_step, int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, void *stream) {
+  if (!ctx) {
+    set_error("nrg_cfs_bin: NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return cfs_run(ctx, pairs, npairs, grid, T, Z, last_step, sign, gt_pos, gt_neg, ls_pos, ls_neg, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("nrg_cfs_bin: ") + e.what());
+    return NRG_E_HIP;
+  }
+}
+
+int nrg_ft_bin(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z,
+               int sign, double *out_pos, double *out_neg, void *stream) {
+  if (!ctx) {
+    set_error("nrg_ft_bin: NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return cfs_run(ctx, pairs, npairs, grid, T, Z, 2, sign, out_pos, out_neg, nullptr, nullptr, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("nrg_ft_bin: ") + e.what());
+    return NRG_E_HIP;
+  }
+}
+
+static int mats_entry(const char *who, nrg_ctx *ctx, int mode, const nrg_fdm_pair *pairs, int npairs, int64_t mats,
+                      double T, double wnf, double Z, int sign, double *G, void *stream) {
+  if (!ctx) {
+    set_error(std::string(who) + ": NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return mats_run(ctx, mode, pairs, npairs, mats, T, wnf, Z, sign, G, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string(who) + ": " + e.what());
+    return NRG_E_HIP;
+  }
+}
+
+int nrg_fdm_mats(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double wnf, int sign,
+                 double *G, void *stream) {
+  return mats_entry("nrg_fdm_mats", ctx, 0, pairs, npairs, mats, T, wnf, 1.0, sign, G, stream);
+}
+
+int nrg_dmnrg_mats(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, int sign, double *G,
+                   void *stream) {
+  return mats_entry("nrg_dmnrg_mats", ctx, 1, pairs, npairs, mats, T, 1.0, 1.0, sign, G, stream);
+}
+
+int nrg_ft_mats(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double Z, int sign,
+                double *G, void *stream) {
+  return mats_entry("nrg_ft_mats", ctx, 2, pairs, npairs, mats, T, 1.0, Z, sign, G, stream);
+}
+
+int nrg_temp_sum(nrg_ctx *ctx, int kind, const nrg_fdm_pair *pairs, int npairs, double temperature, double Z,
+                 double *out, void *stream) {
+  if (!ctx) {
+    set_error("nrg_temp_sum: NULL context");
+    return NRG_E_ARG;
+  }
+  try {
+    NRG_HIP(hipSetDevice(ctx->device));
+    return temp_run(ctx, kind, pairs, npairs, temperature, Z, out, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("nrg_temp_sum: ") + e.what());
+    return NRG_E_HIP;
+  }
+}
+
 int nrg_broaden(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
                 const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
                 double *out_pos, double *out_neg, void *stream) {

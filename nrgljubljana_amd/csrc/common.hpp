@@ -133,6 +133,12 @@ int dmnrg_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin
               int sign, double *out_pos, double *out_neg, hipStream_t st);
 int hamiltonian_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const nrg_ham_term *terms,
                     int nterms, hipStream_t st);
+int cfs_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z, int last,
+            int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, hipStream_t st);
+int mats_run(nrg_ctx *ctx, int mode, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double wnf,
+             double Z, int sign, double *G, hipStream_t st);
+int temp_run(nrg_ctx *ctx, int kind, const nrg_fdm_pair *pairs, int npairs, double temperature, double Z, double *out,
+             hipStream_t st);
 int broaden_run(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e,
                 const double *neg_w, int64_t nneg, const double *mesh, int64_t nmesh, double alpha, double omega0,
                 double *out_pos, double *out_neg, hipStream_t st);

@@ -17,19 +17,15 @@
 // nrg_dmnrg_bin (dmnrg_run, k_dmnrg_bin) does the same for Algo_DMNRG::calc (c++/algo_DMNRG.hpp:30-52):
 // the kept x kept elements of every pair, with the two density-matrix products formed on the matrix
 // cores and an energy window instead of Boltzmann factors.
-#include "gemm.hpp"
+//
+// nrg_cfs_bin (cfs_run, k_cfs_bin) does it for Algo_CFS::calc (c++/algo_CFS.hpp:13-144): at the last
+// step the kept x kept elements with Boltzmann factors, otherwise the discarded x kept (ls) and
+// kept x discarded (gt) elements with the density-matrix products op2 rho_i and rho_j^T op1.
+#include "spectral.hpp"
 #include <cmath>
 #include <cstring>
 
 namespace nrg {
-
-struct FdmPairDev {
-  int64_t dim_i, dim_j, K_i, K_j, ld_op;
-  const double *E_i, *E_j, *op1, *op2;
-  const double *op2rho;   // (dim_j - K_j) x K_i, row-major, ld K_i:  (op2 rho_i)(K_j + r, c)
-  const double *rhoop2;   // K_j x (dim_i - K_i), row-major, ld dim_i - K_i:  (rho_j op2)(r, K_i + c)
-  double factor;
-};
 
 struct FdmArgs {
   const FdmPairDev *pairs;
@@ -72,11 +68,8 @@ __device__ __forceinline__ void bin_pos(double energy, double log10emin, double 
   low = false;
 }
 
-// The scatter goes to FDM_COPIES replicas of the four histograms (workgroup w adds into replica
-// w mod FDM_COPIES): hot bins receive thousands of additions per step, and replicas spread the
-// read-modify-writes on one address over many addresses. k_fdm_reduce then adds the replicas into
-// the caller's arrays.
-constexpr int FDM_COPIES = 32;
+// The scatter goes to FDM_COPIES replicas of the histograms; k_fdm_reduce then adds the replicas
+// into the caller's arrays.
 
 __global__ __launch_bounds__(256) void k_fdm_zero(double *p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0;
@@ -94,6 +87,106 @@ __global__ __launch_bounds__(256) void k_fdm_reduce(const double *priv, int64_t 
   for (int c = 0; c < FDM_COPIES; c++) s += priv[((size_t)c * nh + h) * nbins + b];
   double *out = h == 0 ? o0 : h == 1 ? o1 : h == 2 ? o2 : o3;
   out[b] += s;
+}
+
+void replicas_zero(nrg_ctx *ctx, hipStream_t st, double *p, size_t n) {
+  NRG_TIMED(ctx, st, "k_fdm_zero", hipLaunchKernelGGL(k_fdm_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)),
+                                                      dim3(256), 0, st, p, n));
+}
+
+void replicas_reduce(nrg_ctx *ctx, hipStream_t st, const double *priv, int64_t n, int nh, int nout, double *o0,
+                     double *o1, double *o2, double *o3) {
+  NRG_TIMED(ctx, st, "k_fdm_reduce", hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((n + 255) / 256), nout), dim3(256),
+                                                        0, st, priv, n, nh, o0, o1, o2, o3));
+}
+
+int check_pairs(const char *who, const nrg_fdm_pair *pairs, int npairs, RhoUse use) {
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair &q = pairs[p];
+    if (q.dim_i < 0 || q.dim_j < 0 || q.K_i < 0 || q.K_j < 0 || q.K_i > q.dim_i || q.K_j > q.dim_j ||
+        (q.dim_i && q.dim_j && (!q.op1 || !q.op2 || !q.E_i || !q.E_j || q.ld_op < q.dim_i))) {
+      set_error(std::string(who) + ": pair " + std::to_string(p) + " is inconsistent");
+      return NRG_E_ARG;
+    }
+    if (q.dim_i > INT32_MAX / 2 || q.dim_j > INT32_MAX / 2 || q.ld_op > INT32_MAX || q.n_rho_i > INT32_MAX ||
+        q.n_rho_j > INT32_MAX) {
+      set_error(std::string(who) + ": dimension too large");
+      return NRG_E_ARG;
+    }
+    bool ok = true;
+    if (use == RhoUse::fdm || use == RhoUse::cfs) {
+      // the products need rho on every kept state they touch
+      const bool need_i = q.K_i > 0 && q.dim_j > q.K_j, need_j = q.K_j > 0 && q.dim_i > q.K_i;
+      ok = !(need_i && (!q.rho_i || q.n_rho_i < q.K_i)) && !(need_j && (!q.rho_j || q.n_rho_j < q.K_j));
+      if (use == RhoUse::cfs)   // prod_fit_left / prod_adj_fit_left: rho within the subspace
+        ok = ok && !(need_i && q.n_rho_i > q.dim_i) && !(need_j && q.n_rho_j > q.dim_j);
+    } else if (use == RhoUse::dmnrg) {
+      ok = !(q.K_i && q.K_j && (!q.rho_i || q.n_rho_i < q.K_i || !q.rho_j || q.n_rho_j < q.K_j));
+    }
+    if (!ok) {
+      set_error(std::string(who) + ": pair " + std::to_string(p) + " needs density matrices covering its kept states");
+      return NRG_E_ARG;
+    }
+  }
+  return NRG_OK;
+}
+
+int64_t fdm_products(const nrg_fdm_pair &q, double *buf, FdmPairDev &d, std::vector<GemmDesc> &gd) {
+  d = FdmPairDev{};
+  d.dim_i = q.dim_i; d.dim_j = q.dim_j; d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
+  d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
+  const int64_t n_or = q.K_i > 0 && q.dim_j > q.K_j ? (q.dim_j - q.K_j) * q.K_i : 0;
+  const int64_t n_ro = q.K_j > 0 && q.dim_i > q.K_i ? q.K_j * (q.dim_i - q.K_i) : 0;
+  if (!buf) return n_or + n_ro;
+  if (n_or) {
+    d.op2rho = buf;
+    // (op2 rho_i)[K_j:, :K_i] = op2[K_j:, :c] rho_i[:c, :K_i], c = min(n_rho_i, dim_i) (prod_fit)
+    const int64_t c = std::min(q.n_rho_i, q.dim_i);
+    // row-major C (R x K_i) = X (R x c) * Y (c x K_i): column-major C^T = Y^T X^T
+    GemmDesc g{};
+    g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 0;
+    g.B = q.op2 + (size_t)q.K_j * q.ld_op; g.ldb = (int)q.ld_op; g.tB = 0;
+    g.C = buf; g.ldc = (int)q.K_i;
+    g.M = (int)q.K_i; g.N = (int)(q.dim_j - q.K_j); g.K = (int)c; g.alpha = 1.0; g.beta = 0.0;
+    gd.push_back(g);
+  }
+  if (n_ro) {
+    d.rhoop2 = buf + n_or;
+    // (rho_j op2)[:K_j, K_i:] = rho_j[:K_j, :c] op2[:c, K_i:], c = min(n_rho_j, dim_j)
+    const int64_t c = std::min(q.n_rho_j, q.dim_j);
+    GemmDesc g{};
+    g.A = q.op2 + q.K_i; g.lda = (int)q.ld_op; g.tA = 0;
+    g.B = q.rho_j; g.ldb = (int)q.n_rho_j; g.tB = 0;
+    g.C = buf + n_or; g.ldc = (int)(q.dim_i - q.K_i);
+    g.M = (int)(q.dim_i - q.K_i); g.N = (int)q.K_j; g.K = (int)c; g.alpha = 1.0; g.beta = 0.0;
+    gd.push_back(g);
+  }
+  return n_or + n_ro;
+}
+
+int64_t dmnrg_products(const nrg_fdm_pair &q, double *buf, DmnrgPairDev &d, std::vector<GemmDesc> &gd) {
+  d = DmnrgPairDev{};
+  d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
+  d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
+  const int64_t n = q.K_i > 0 && q.K_j > 0 ? 2 * q.K_i * q.K_j : 0;
+  if (!buf || !n) return n;
+  double *pa = buf, *pb = buf + q.K_i * q.K_j;
+  d.pa = pa; d.pb = pb;
+  // row-major PA (K_j x K_i) = op2[:K_j, :K_i] rho_i[:K_i, :K_i]^T: column-major PA^T = rho_i op2s^T
+  GemmDesc g{};
+  g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 1;
+  g.B = q.op2; g.ldb = (int)q.ld_op; g.tB = 0;
+  g.C = pa; g.ldc = (int)q.K_i;
+  g.M = (int)q.K_i; g.N = (int)q.K_j; g.K = (int)q.K_i; g.alpha = 1.0; g.beta = 0.0;
+  gd.push_back(g);
+  // row-major PB (K_j x K_i) = rho_j[:K_j, :K_j] op1[:K_j, :K_i]: column-major PB^T = op1s^T rho_j^T
+  GemmDesc h{};
+  h.A = q.op1; h.lda = (int)q.ld_op; h.tA = 0;
+  h.B = q.rho_j; h.ldb = (int)q.n_rho_j; h.tB = 0;
+  h.C = pb; h.ldc = (int)q.K_i;
+  h.M = (int)q.K_i; h.N = (int)q.K_j; h.K = (int)q.K_j; h.alpha = 1.0; h.beta = 0.0;
+  gd.push_back(h);
+  return n;
 }
 
 __global__ __launch_bounds__(256) void k_fdm_bin(FdmArgs a, const int2 *items) {
@@ -163,135 +256,47 @@ int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_g
     set_error("nrg_fdm_bin: more than 2^20 bins");
     return NRG_E_ARG;
   }
-  for (int p = 0; p < npairs; p++) {
-    const nrg_fdm_pair &q = pairs[p];
-    if (q.dim_i < 0 || q.dim_j < 0 || q.K_i < 0 || q.K_j < 0 || q.K_i > q.dim_i || q.K_j > q.dim_j ||
-        (q.dim_i && q.dim_j && (!q.op1 || !q.op2 || !q.E_i || !q.E_j || q.ld_op < q.dim_i))) {
-      set_error("nrg_fdm_bin: pair " + std::to_string(p) + " is inconsistent");
-      return NRG_E_ARG;
-    }
-    if (q.dim_i > INT32_MAX / 2 || q.dim_j > INT32_MAX / 2 || q.ld_op > INT32_MAX || q.n_rho_i > INT32_MAX ||
-        q.n_rho_j > INT32_MAX) {
-      set_error("nrg_fdm_bin: dimension too large");
-      return NRG_E_ARG;
-    }
-    // the density-matrix products need rho on every kept state they touch
-    const bool need_i = q.K_i > 0 && q.dim_j > q.K_j, need_j = q.K_j > 0 && q.dim_i > q.K_i;
-    if ((need_i && (!q.rho_i || q.n_rho_i < q.K_i)) || (need_j && (!q.rho_j || q.n_rho_j < q.K_j))) {
-      set_error("nrg_fdm_bin: pair " + std::to_string(p) + " needs a density matrix covering its kept states");
-      return NRG_E_ARG;
-    }
-  }
+  if (int r = check_pairs("nrg_fdm_bin", pairs, npairs, RhoUse::fdm)) return r;
   // product buffers
-  std::vector<int64_t> o_or(npairs, -1), o_ro(npairs, -1);
-  int64_t tot = 0;
-  for (int p = 0; p < npairs; p++) {
-    const nrg_fdm_pair &q = pairs[p];
-    if (q.K_i > 0 && q.dim_j > q.K_j) { o_or[p] = tot; tot += (q.dim_j - q.K_j) * q.K_i; }
-    if (q.K_j > 0 && q.dim_i > q.K_i) { o_ro[p] = tot; tot += q.K_j * (q.dim_i - q.K_i); }
-  }
-  if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
-  if (int r = ctx->ws.ensure(sizeof(double) * (size_t)FDM_COPIES * 4 * (size_t)grid->nbins)) return r;
-  double *prod = ctx->ws2.as<double>();
   std::vector<GemmDesc> gd;
   std::vector<FdmPairDev> pd(npairs);
+  int64_t tot = 0;
+  for (int p = 0; p < npairs; p++) tot += fdm_products(pairs[p], nullptr, pd[p], gd);
+  if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
+  double *prod = ctx->ws2.as<double>();
   std::vector<int> items;
-  for (int p = 0; p < npairs; p++) {
+  for (int64_t p = 0, off = 0; p < npairs; p++) {
     const nrg_fdm_pair &q = pairs[p];
-    FdmPairDev &d = pd[p];
-    d.dim_i = q.dim_i; d.dim_j = q.dim_j; d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
-    d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
-    d.op2rho = o_or[p] >= 0 ? prod + o_or[p] : nullptr;
-    d.rhoop2 = o_ro[p] >= 0 ? prod + o_ro[p] : nullptr;
-    if (o_or[p] >= 0) {
-      // (op2 rho_i)[K_j:, :K_i] = op2[K_j:, :c] rho_i[:c, :K_i], c = min(n_rho_i, dim_i) (prod_fit)
-      const int64_t c = std::min(q.n_rho_i, q.dim_i);
-      // row-major C (R x K_i) = X (R x c) * Y (c x K_i): column-major C^T = Y^T X^T
-      GemmDesc g{};
-      g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 0;
-      g.B = q.op2 + (size_t)q.K_j * q.ld_op; g.ldb = (int)q.ld_op; g.tB = 0;
-      g.C = (double *)d.op2rho; g.ldc = (int)q.K_i;
-      g.M = (int)q.K_i; g.N = (int)(q.dim_j - q.K_j); g.K = (int)c; g.alpha = 1.0; g.beta = 0.0;
-      gd.push_back(g);
-    }
-    if (o_ro[p] >= 0) {
-      // (rho_j op2)[:K_j, K_i:] = rho_j[:K_j, :c] op2[:c, K_i:], c = min(n_rho_j, dim_j)
-      const int64_t c = std::min(q.n_rho_j, q.dim_j);
-      GemmDesc g{};
-      g.A = q.op2 + q.K_i; g.lda = (int)q.ld_op; g.tA = 0;
-      g.B = q.rho_j; g.ldb = (int)q.n_rho_j; g.tB = 0;
-      g.C = (double *)d.rhoop2; g.ldc = (int)(q.dim_i - q.K_i);
-      g.M = (int)(q.dim_i - q.K_i); g.N = (int)q.K_j; g.K = (int)c; g.alpha = 1.0; g.beta = 0.0;
-      gd.push_back(g);
-    }
+    off += fdm_products(q, prod + off, pd[p], gd);
     // element tiles, skipping those that lie entirely in K_i x K_j
     const int ti = (int)((q.dim_i + 63) / 64), tj = (int)((q.dim_j + 63) / 64);
     for (int b = 0; b < tj; b++)
       for (int a = 0; a < ti; a++) {
         const bool allK = (int64_t)(a + 1) * 64 <= q.K_i && (int64_t)(b + 1) * 64 <= q.K_j;
         if (allK) continue;
-        items.push_back(p);
+        items.push_back((int)p);
         items.push_back(b * ti + a);
       }
   }
-  std::vector<int> gitems;
-  gemm_items(gd, 0, gd.size(), gitems);
-  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t bG = sizeof(GemmDesc) * gd.size(), bPd = sizeof(FdmPairDev) * pd.size();
-  const size_t oG = 0, oPd = al(bG), oGI = oPd + al(bPd), oI = oGI + al(sizeof(int) * gitems.size());
-  const size_t total = oI + sizeof(int) * items.size() + 8;
-  std::vector<char> blob(total);
-  if (bG) memcpy(blob.data() + oG, gd.data(), bG);
-  if (bPd) memcpy(blob.data() + oPd, pd.data(), bPd);
-  if (!gitems.empty()) memcpy(blob.data() + oGI, gitems.data(), sizeof(int) * gitems.size());
-  if (!items.empty()) memcpy(blob.data() + oI, items.data(), sizeof(int) * items.size());
-  NRG_HIP(hipEventRecord(ctx->ev[0], st));
-  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
-  if (!gitems.empty())
-    gemm_launch(ctx, st, (const GemmDesc *)ctx->desc.as<char>(oG), ctx->desc.as<int>(oGI), gitems.data(), gitems.size() / 2);
-  NRG_HIP(hipEventRecord(ctx->ev[1], st));
-  FdmArgs fa;
-  fa.pairs = (const FdmPairDev *)ctx->desc.as<char>(oPd);
-  fa.T = T; fa.wnf = wnf; fa.msign = -(double)sign;
-  fa.log10emin = grid->log10emin; fa.bpd = grid->bins_per_decade; fa.discard = grid->discard_immediately;
-  fa.nbins = grid->nbins;
-  double *priv = ctx->ws.as<double>();   // FDM_COPIES x 4 histograms
-  fa.gt_pos = priv; fa.gt_neg = priv; fa.ls_pos = priv; fa.ls_neg = priv;   // offset per replica in the kernel
-  if (!items.empty()) {
-    const size_t np = (size_t)FDM_COPIES * 4 * grid->nbins;
-    NRG_TIMED(ctx, st, "k_fdm_zero", hipLaunchKernelGGL(k_fdm_zero, dim3((unsigned)std::min<size_t>((np + 255) / 256, 4096)),
-                                                        dim3(256), 0, st, priv, np));
+  if (int r = ctx->ws.ensure(sizeof(double) * (size_t)FDM_COPIES * 4 * (size_t)grid->nbins)) return r;
+  return run_spectral_step(ctx, st, gd, pd, items, [&](const FdmPairDev *pdev, const int2 *idev) {
+    FdmArgs fa;
+    fa.pairs = pdev;
+    fa.T = T; fa.wnf = wnf; fa.msign = -(double)sign;
+    fa.log10emin = grid->log10emin; fa.bpd = grid->bins_per_decade; fa.discard = grid->discard_immediately;
+    fa.nbins = grid->nbins;
+    double *priv = ctx->ws.as<double>();   // FDM_COPIES x 4 histograms
+    fa.gt_pos = priv; fa.gt_neg = priv; fa.ls_pos = priv; fa.ls_neg = priv;   // offset per replica in the kernel
+    replicas_zero(ctx, st, priv, (size_t)FDM_COPIES * 4 * grid->nbins);
     NRG_TIMED(ctx, st, "k_fdm_bin", hipLaunchKernelGGL(k_fdm_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st, fa,
-                                                       (const int2 *)ctx->desc.as<char>(oI)));
-    NRG_TIMED(ctx, st, "k_fdm_reduce", hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((grid->nbins + 255) / 256), 4),
-                                                          dim3(256), 0, st, priv, (int64_t)grid->nbins, 4, gt_pos,
-                                                          gt_neg, ls_pos, ls_neg));
-  }
-  NRG_HIP(hipGetLastError());
-  NRG_HIP(hipEventRecord(ctx->ev[2], st));
-  NRG_HIP(hipStreamSynchronize(st));
-  float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]);
-  ctx->total_ms = ms;
-  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-  ctx->phase_ms[0] = ms;
-  hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
-  ctx->phase_ms[1] = ms;
-  if (ctx->kt.on) ctx->kt.collect();
-  return NRG_OK;
+                                                       idev));
+    replicas_reduce(ctx, st, priv, grid->nbins, 4, 4, gt_pos, gt_neg, ls_pos, ls_neg);
+  });
 }
 
 // ------------------------------------------------------------------------------------------
 // DM-NRG (Algo_DMNRG::calc, c++/algo_DMNRG.hpp:30-52)
 // ------------------------------------------------------------------------------------------
-struct DmnrgPairDev {
-  int64_t K_i, K_j, ld_op;
-  const double *E_i, *E_j, *op1, *op2;
-  const double *pa;   // K_j x K_i row-major: (op2 rho_i^T)(j, m) over the kept states
-  const double *pb;   // K_j x K_i row-major: (rho_j op1)(j, m)
-  double factor;
-};
-
 struct DmnrgArgs {
   const DmnrgPairDev *pairs;
   double emin, emax, msign;
@@ -347,55 +352,21 @@ int dmnrg_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin
     set_error("nrg_dmnrg_bin: more than 2^20 bins");
     return NRG_E_ARG;
   }
-  for (int p = 0; p < npairs; p++) {
-    const nrg_fdm_pair &q = pairs[p];
-    if (q.dim_i < 0 || q.dim_j < 0 || q.K_i < 0 || q.K_j < 0 || q.K_i > q.dim_i || q.K_j > q.dim_j ||
-        (q.K_i && q.K_j && (!q.op1 || !q.op2 || !q.E_i || !q.E_j || q.ld_op < q.dim_i))) {
-      set_error("nrg_dmnrg_bin: pair " + std::to_string(p) + " is inconsistent");
-      return NRG_E_ARG;
-    }
-    if (q.dim_i > INT32_MAX / 2 || q.dim_j > INT32_MAX / 2 || q.ld_op > INT32_MAX || q.n_rho_i > INT32_MAX ||
-        q.n_rho_j > INT32_MAX) {
-      set_error("nrg_dmnrg_bin: dimension too large");
-      return NRG_E_ARG;
-    }
-    if (q.K_i && q.K_j && (!q.rho_i || q.n_rho_i < q.K_i || !q.rho_j || q.n_rho_j < q.K_j)) {
-      set_error("nrg_dmnrg_bin: pair " + std::to_string(p) + " needs density matrices covering its kept states");
-      return NRG_E_ARG;
-    }
-  }
-  std::vector<int64_t> off(npairs, -1);
+  if (int r = check_pairs("nrg_dmnrg_bin", pairs, npairs, RhoUse::dmnrg)) return r;
+  std::vector<GemmDesc> gd;
+  std::vector<DmnrgPairDev> pd;
+  DmnrgPairDev scratch;
   int64_t tot = 0;
-  for (int p = 0; p < npairs; p++)
-    if (pairs[p].K_i > 0 && pairs[p].K_j > 0) { off[p] = tot; tot += 2 * pairs[p].K_i * pairs[p].K_j; }
+  for (int p = 0; p < npairs; p++) tot += dmnrg_products(pairs[p], nullptr, scratch, gd);
   if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
   if (int r = ctx->ws.ensure(sizeof(double) * (size_t)FDM_COPIES * 2 * (size_t)grid->nbins)) return r;
   double *prod = ctx->ws2.as<double>();
-  std::vector<GemmDesc> gd;
-  std::vector<DmnrgPairDev> pd;
   std::vector<int> items;
-  for (int p = 0; p < npairs; p++) {
-    if (off[p] < 0) continue;
+  for (int64_t p = 0, off = 0; p < npairs; p++) {
     const nrg_fdm_pair &q = pairs[p];
-    DmnrgPairDev d{};
-    d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
-    d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
-    double *pa = prod + off[p], *pb = pa + q.K_i * q.K_j;
-    d.pa = pa; d.pb = pb;
-    // row-major PA (K_j x K_i) = op2[:K_j, :K_i] rho_i[:K_i, :K_i]^T: column-major PA^T = rho_i op2s^T
-    GemmDesc g{};
-    g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 1;
-    g.B = q.op2; g.ldb = (int)q.ld_op; g.tB = 0;
-    g.C = pa; g.ldc = (int)q.K_i;
-    g.M = (int)q.K_i; g.N = (int)q.K_j; g.K = (int)q.K_i; g.alpha = 1.0; g.beta = 0.0;
-    gd.push_back(g);
-    // row-major PB (K_j x K_i) = rho_j[:K_j, :K_j] op1[:K_j, :K_i]: column-major PB^T = op1s^T rho_j^T
-    GemmDesc h{};
-    h.A = q.op1; h.lda = (int)q.ld_op; h.tA = 0;
-    h.B = q.rho_j; h.ldb = (int)q.n_rho_j; h.tB = 0;
-    h.C = pb; h.ldc = (int)q.K_i;
-    h.M = (int)q.K_i; h.N = (int)q.K_j; h.K = (int)q.K_j; h.alpha = 1.0; h.beta = 0.0;
-    gd.push_back(h);
+    if (!(q.K_i > 0 && q.K_j > 0)) continue;
+    DmnrgPairDev d;
+    off += dmnrg_products(q, prod + off, d, gd);
     const int ti = (int)((q.K_i + 63) / 64), tj = (int)((q.K_j + 63) / 64);
     for (int t = 0; t < ti * tj; t++) {
       items.push_back((int)pd.size());
@@ -403,50 +374,173 @@ int dmnrg_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin
     }
     pd.push_back(d);
   }
-  std::vector<int> gitems;
-  gemm_items(gd, 0, gd.size(), gitems);
-  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t bG = sizeof(GemmDesc) * gd.size(), bPd = sizeof(DmnrgPairDev) * pd.size();
-  const size_t oG = 0, oPd = al(bG), oGI = oPd + al(bPd), oI = oGI + al(sizeof(int) * gitems.size());
-  const size_t total = oI + sizeof(int) * items.size() + 8;
-  std::vector<char> blob(total);
-  if (bG) memcpy(blob.data() + oG, gd.data(), bG);
-  if (bPd) memcpy(blob.data() + oPd, pd.data(), bPd);
-  if (!gitems.empty()) memcpy(blob.data() + oGI, gitems.data(), sizeof(int) * gitems.size());
-  if (!items.empty()) memcpy(blob.data() + oI, items.data(), sizeof(int) * items.size());
-  NRG_HIP(hipEventRecord(ctx->ev[0], st));
-  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
-  if (!gitems.empty())
-    gemm_launch(ctx, st, (const GemmDesc *)ctx->desc.as<char>(oG), ctx->desc.as<int>(oGI), gitems.data(), gitems.size() / 2);
-  NRG_HIP(hipEventRecord(ctx->ev[1], st));
-  DmnrgArgs da;
-  da.pairs = (const DmnrgPairDev *)ctx->desc.as<char>(oPd);
-  da.emin = emin; da.emax = emax; da.msign = -(double)sign;
-  da.log10emin = grid->log10emin; da.bpd = grid->bins_per_decade; da.discard = grid->discard_immediately;
-  da.nbins = grid->nbins;
-  da.priv = ctx->ws.as<double>();
-  if (!items.empty()) {
-    const size_t np = (size_t)FDM_COPIES * 2 * grid->nbins;
-    NRG_TIMED(ctx, st, "k_fdm_zero", hipLaunchKernelGGL(k_fdm_zero, dim3((unsigned)std::min<size_t>((np + 255) / 256, 4096)),
-                                                        dim3(256), 0, st, da.priv, np));
+  return run_spectral_step(ctx, st, gd, pd, items, [&](const DmnrgPairDev *pdev, const int2 *idev) {
+    DmnrgArgs da;
+    da.pairs = pdev;
+    da.emin = emin; da.emax = emax; da.msign = -(double)sign;
+    da.log10emin = grid->log10emin; da.bpd = grid->bins_per_decade; da.discard = grid->discard_immediately;
+    da.nbins = grid->nbins;
+    da.priv = ctx->ws.as<double>();
+    replicas_zero(ctx, st, da.priv, (size_t)FDM_COPIES * 2 * grid->nbins);
     NRG_TIMED(ctx, st, "k_dmnrg_bin", hipLaunchKernelGGL(k_dmnrg_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st,
-                                                         da, (const int2 *)ctx->desc.as<char>(oI)));
-    NRG_TIMED(ctx, st, "k_fdm_reduce", hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((grid->nbins + 255) / 256), 2),
-                                                          dim3(256), 0, st, da.priv, (int64_t)grid->nbins, 2, out_pos,
-                                                          out_neg, out_pos, out_neg));
+                                                         da, idev));
+    replicas_reduce(ctx, st, da.priv, grid->nbins, 2, 2, out_pos, out_neg, out_pos, out_neg);
+  });
+}
+
+// ------------------------------------------------------------------------------------------
+// CFS (Algo_CFS::calc = Algo_CFSgt::calc + Algo_CFSls::calc, c++/algo_CFS.hpp:29-56, 79-106)
+// ------------------------------------------------------------------------------------------
+struct CfsArgs {
+  const FdmPairDev *pairs;   // op2rho: (op2 rho_i)[K_j:, :K_i]; rhoop2: (rho_j^T op1)[:K_j, K_i:], ld dim_i - K_i
+  double T, Z, msign;
+  double log10emin, bpd, discard;
+  int64_t nbins;
+  int last;       // 0: below the last step, 1: the last step, 2: Algo_FT (kept x kept into the gt pair)
+  double *priv;   // FDM_COPIES replicas of (gt_pos, gt_neg, ls_pos, ls_neg)
+};
+
+__global__ __launch_bounds__(256) void k_cfs_bin(CfsArgs a, const int2 *items) {
+  const int2 it = items[blockIdx.x];
+  const FdmPairDev P = a.pairs[it.x];
+  double *gp = a.priv + (size_t)(blockIdx.x % FDM_COPIES) * 4 * a.nbins;
+  double *gn = gp + a.nbins, *lp = gn + a.nbins, *ln = lp + a.nbins;
+  const int ti = (int)((P.dim_i + 63) / 64);
+  const int i0 = (it.y % ti) * 64, j0 = (it.y / ti) * 64;
+  __shared__ double Ei[64], Ej[64], Bi[64], Bj[64];
+  if (threadIdx.x < 64) {
+    const int i = i0 + threadIdx.x;
+    const double e = i < P.dim_i ? P.E_i[i] : 0.0;
+    Ei[threadIdx.x] = e;
+    Bi[threadIdx.x] = a.last ? exp(-e / a.T) : 0.0;
+  } else if (threadIdx.x < 128) {
+    const int j = j0 + threadIdx.x - 64;
+    const double e = j < P.dim_j ? P.E_j[j] : 0.0;
+    Ej[threadIdx.x - 64] = e;
+    Bj[threadIdx.x - 64] = a.last ? exp(-e / a.T) : 0.0;
   }
-  NRG_HIP(hipGetLastError());
-  NRG_HIP(hipEventRecord(ctx->ev[2], st));
-  NRG_HIP(hipStreamSynchronize(st));
-  float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]);
-  ctx->total_ms = ms;
-  hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-  ctx->phase_ms[0] = ms;
-  hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
-  ctx->phase_ms[1] = ms;
-  if (ctx->kt.on) ctx->kt.collect();
-  return NRG_OK;
+  __syncthreads();
+  const int il = threadIdx.x & 63;
+  const int i = i0 + il;
+  if (i >= P.dim_i) return;
+  const bool ki = i < P.K_i;
+  for (int jl = threadIdx.x >> 6; jl < 64; jl += 4) {
+    const int j = j0 + jl;
+    if (j >= P.dim_j) break;
+    const bool kj = j < P.K_j;
+    const size_t e = (size_t)j * P.ld_op + i;
+    double wg = 0.0, wl = 0.0;
+    if (a.last == 2) {   // FT: kept x kept, ((-sign) exp(-E_j/T) + exp(-E_i/T)) / Z, one histogram pair
+      if (!(ki && kj)) continue;
+      wg = P.op1[e] * P.op2[e] * ((a.msign * Bj[jl] + Bi[il]) / a.Z);
+    } else if (a.last) {   // Eq. (11), kept x kept
+      if (!(ki && kj)) continue;
+      const double oo = P.op1[e] * P.op2[e];
+      wg = oo * Bi[il] / a.Z;
+      wl = oo * Bj[jl] * a.msign / a.Z;
+    } else if (ki && !kj) {   // Eq. (16), discarded j x kept i
+      wl = P.op1[e] * P.op2rho[(size_t)(j - P.K_j) * P.K_i + i] * a.msign;
+    } else if (!ki && kj) {   // Eq. (15), kept j x discarded i
+      wg = P.rhoop2[(size_t)j * (P.dim_i - P.K_i) + (i - P.K_i)] * P.op2[e];
+    } else {
+      continue;
+    }
+    const double de = Ej[jl] - Ei[il];
+    const double energy = fabs(de);
+    int64_t idx;
+    double rem;
+    bool low;
+    bin_pos(energy, a.log10emin, a.bpd, a.nbins, idx, rem, low);
+    bin_add(a.discard, de >= 0.0 ? gp : gn, energy, wg * P.factor, idx, rem, low);
+    bin_add(a.discard, de >= 0.0 ? lp : ln, energy, wl * P.factor, idx, rem, low);
+  }
+}
+
+// last: 0 / 1 as step.last() for Algo_CFS; 2 for Algo_FT (gt_pos / gt_neg receive the weights, the
+// lesser pair is not touched and may be null)
+int cfs_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z, int last,
+            int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, hipStream_t st) {
+  const char *who = last == 2 ? "nrg_ft_bin" : "nrg_cfs_bin";
+  if (npairs < 0 || (npairs > 0 && !pairs) || !grid || grid->nbins < 2 || (sign != 1 && sign != -1) || !gt_pos ||
+      !gt_neg || (last != 2 && (!ls_pos || !ls_neg)) || !(grid->bins_per_decade > 0) || (last && !(T > 0 && Z > 0))) {
+    set_error(std::string(who) + ": invalid arguments");
+    return NRG_E_ARG;
+  }
+  if (grid->nbins > (1 << 20)) {
+    set_error(std::string(who) + ": more than 2^20 bins");
+    return NRG_E_ARG;
+  }
+  if (int r = check_pairs(who, pairs, npairs, last ? RhoUse::none : RhoUse::cfs)) return r;
+  std::vector<int64_t> o_or(npairs, -1), o_ro(npairs, -1);
+  int64_t tot = 0;
+  if (!last)
+    for (int p = 0; p < npairs; p++) {
+      const nrg_fdm_pair &q = pairs[p];
+      if (q.K_i > 0 && q.dim_j > q.K_j) { o_or[p] = tot; tot += (q.dim_j - q.K_j) * q.K_i; }
+      if (q.K_j > 0 && q.dim_i > q.K_i) { o_ro[p] = tot; tot += q.K_j * (q.dim_i - q.K_i); }
+    }
+  if (int r = ctx->ws2.ensure(sizeof(double) * (size_t)(tot + 1))) return r;
+  if (int r = ctx->ws.ensure(sizeof(double) * (size_t)FDM_COPIES * 4 * (size_t)grid->nbins)) return r;
+  double *prod = ctx->ws2.as<double>();
+  std::vector<GemmDesc> gd;
+  std::vector<FdmPairDev> pd;
+  std::vector<int> items;
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair &q = pairs[p];
+    FdmPairDev d{};
+    d.dim_i = q.dim_i; d.dim_j = q.dim_j; d.K_i = q.K_i; d.K_j = q.K_j; d.ld_op = q.ld_op;
+    d.E_i = q.E_i; d.E_j = q.E_j; d.op1 = q.op1; d.op2 = q.op2; d.factor = q.factor;
+    d.op2rho = o_or[p] >= 0 ? prod + o_or[p] : nullptr;
+    d.rhoop2 = o_ro[p] >= 0 ? prod + o_ro[p] : nullptr;
+    if (o_or[p] >= 0) {
+      // row-major (op2 rho_i)[K_j:, :K_i] = op2[K_j:, :n] rho_i[:n, :K_i]: column-major rho_i^T-view times op2^T-view
+      GemmDesc g{};
+      g.A = q.rho_i; g.lda = (int)q.n_rho_i; g.tA = 0;
+      g.B = q.op2 + (size_t)q.K_j * q.ld_op; g.ldb = (int)q.ld_op; g.tB = 0;
+      g.C = (double *)d.op2rho; g.ldc = (int)q.K_i;
+      g.M = (int)q.K_i; g.N = (int)(q.dim_j - q.K_j); g.K = (int)q.n_rho_i; g.alpha = 1.0; g.beta = 0.0;
+      gd.push_back(g);
+    }
+    if (o_ro[p] >= 0) {
+      // row-major X = (rho_j^T op1)[:K_j, K_i:], X(r, c) = sum_m rho_j(m, r) op1(m, K_i + c), m < n_rho_j:
+      // column-major X^T = op1[:n, K_i:]-view (M x n) times rho_j (n x K_j, stored row-major: transposed)
+      GemmDesc g{};
+      g.A = q.op1 + q.K_i; g.lda = (int)q.ld_op; g.tA = 0;
+      g.B = q.rho_j; g.ldb = (int)q.n_rho_j; g.tB = 1;
+      g.C = (double *)d.rhoop2; g.ldc = (int)(q.dim_i - q.K_i);
+      g.M = (int)(q.dim_i - q.K_i); g.N = (int)q.K_j; g.K = (int)q.n_rho_j; g.alpha = 1.0; g.beta = 0.0;
+      gd.push_back(g);
+    }
+    // element tiles of the full dim_i x dim_j grid that meet the elements this step evaluates
+    const int64_t ti = (q.dim_i + 63) / 64, tj = (q.dim_j + 63) / 64;
+    bool any = false;
+    for (int64_t b = 0; b < tj; b++)
+      for (int64_t c = 0; c < ti; c++) {
+        const int64_t a0 = c * 64, a1 = std::min<int64_t>(a0 + 64, q.dim_i);   // i range
+        const int64_t b0 = b * 64, b1 = std::min<int64_t>(b0 + 64, q.dim_j);   // j range
+        const bool kk = a0 < q.K_i && b0 < q.K_j;
+        const bool ls = a0 < q.K_i && b1 > q.K_j, gt = a1 > q.K_i && b0 < q.K_j;
+        if (last ? !kk : !(ls || gt)) continue;
+        items.push_back((int)pd.size());
+        items.push_back((int)(b * ti + c));
+        any = true;
+      }
+    if (any) pd.push_back(d);
+  }
+  return run_spectral_step(ctx, st, gd, pd, items, [&](const FdmPairDev *pdev, const int2 *idev) {
+    CfsArgs ca;
+    ca.pairs = pdev;
+    ca.T = T; ca.Z = Z; ca.msign = -(double)sign;
+    ca.log10emin = grid->log10emin; ca.bpd = grid->bins_per_decade; ca.discard = grid->discard_immediately;
+    ca.nbins = grid->nbins;
+    ca.last = last == 2 ? 2 : last ? 1 : 0;
+    ca.priv = ctx->ws.as<double>();
+    replicas_zero(ctx, st, ca.priv, (size_t)FDM_COPIES * 4 * grid->nbins);
+    NRG_TIMED(ctx, st, "k_cfs_bin", hipLaunchKernelGGL(k_cfs_bin, dim3((unsigned)(items.size() / 2)), dim3(256), 0, st, ca,
+                                                       idev));
+    // replica layout (gt_pos, gt_neg, ls_pos, ls_neg) per copy; FT reduces the first two only
+    replicas_reduce(ctx, st, ca.priv, grid->nbins, 4, last == 2 ? 2 : 4, gt_pos, gt_neg, ls_pos, ls_neg);
+  });
 }
 
 }  // namespace nrg

@@ -135,6 +135,18 @@ def fdm_elements(pairs):
     return sum(p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"] for p in pairs)
 
 
+def cfs_elements(pairs):
+    """Elements a CFS step below the last one evaluates: |K_i||D_j| (Eq. 16) + |D_i||K_j| (Eq. 15)."""
+    return sum(p["K_i"] * (p["dim_j"] - p["K_j"]) + p["K_j"] * (p["dim_i"] - p["K_i"]) for p in pairs)
+
+
+def cfs_flops(pairs):
+    """The two density-matrix products of a CFS step: op2[D_j, :] rho_i (2 |D_j| K_i^2) and
+    rho_j^T op1[:, D_i] (2 |D_i| K_j^2), density matrices over the kept states."""
+    return sum(2.0 * (p["dim_j"] - p["K_j"]) * p["K_i"] ** 2 + 2.0 * (p["dim_i"] - p["K_i"]) * p["K_j"] ** 2
+               for p in pairs)
+
+
 def ham_step(nblk=256, seed=SEED + 6):
     """One step's Hamiltonian blocks with the eigensolver workload's orders (log-uniform [16,1024]):
     each block split into 4 sub-blocks (the combs of a one-channel QS step), the ancestors' energies

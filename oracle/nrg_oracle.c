@@ -20,6 +20,13 @@
  *   oracle_broaden        c++/spectrum.hpp:180-209, c++/spectral.hpp:39-71 (BR_NEW broadening)
  *   oracle_dmnrg_bin      c++/algo_DMNRG.hpp:30-52 (weights lambda and the kept x kept loop, inner
  *                         sums as plain loops), ChainBinning::add as for FDM
+ *   oracle_cfs_bin        c++/algo_CFS.hpp:29-56,79-106,129-133 (CFSgt then CFSls terms, Eqs. 11,
+ *                         15, 16), c++/numerics.hpp:283-316 (prod_fit_left, prod_adj_fit_left),
+ *                         ChainBinning::add as for FDM
+ *   oracle_ft_bin         c++/algo_FT.hpp:28-44 (stat_factor, the kept x kept loop)
+ *   oracle_mats           c++/algo_FDM.hpp:167-213, c++/algo_DMNRG.hpp:74-100, c++/algo_FT.hpp:59-84
+ *                         (the Matsubara terms and loops), c++/matsubara.hpp:20-27 (ww)
+ *   oracle_temp_sum       c++/algo_FT.hpp:95-170 (Algo_GT<n>, Algo_CHIT), c++/numerics.hpp:323-336
  *   oracle_hamiltonian    c++/core.hpp:72-88 (zero matrix, then the terms in order),
  *                         c++/matrix.hpp:32-111 (offdiag_function_impl, diag_function_impl,
  *                         diag_offdiag_function)
@@ -42,6 +49,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include <complex.h>
 #include "../include/nrgcore.h"
 
 typedef void (*dsyev_t)(const char *, const char *, const int *, double *, const int *, double *, double *,
@@ -304,6 +312,223 @@ int oracle_dmnrg_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *
       }
   }
   return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * CFS spectral weights (Algo_CFS::calc = Algo_CFSgt::calc then Algo_CFSls::calc,
+ * c++/algo_CFS.hpp:29-56, 79-106, 129-133). Subspace i plays the reference's Ip, j its I1;
+ * E_i, E_j are the abs_zero energies, rho_i = rho.at(Ip), rho_j = rho.at(I1).
+ * ------------------------------------------------------------------------- */
+int oracle_cfs_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z, int last,
+                   int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg) {
+  if (!p_dgemm) return -1;
+  chain_t gt = {grid, gt_pos, gt_neg}, ls = {grid, ls_pos, ls_neg};
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair *q = &pairs[p];
+    const int64_t di = q->dim_i, dj = q->dim_j, Ki = q->K_i, Kj = q->K_j, ld = q->ld_op;
+    const double *op1 = q->op1, *op2 = q->op2;
+#define OP1(j, i) op1[(j) * ld + (i)]
+#define OP2(j, i) op2[(j) * ld + (i)]
+    /* Algo_CFSgt::calc (c++/algo_CFS.hpp:79-106) */
+    if (last) {
+      for (int64_t r1 = 0; r1 < Kj; r1++)       /* Eq. (11): kept x kept */
+        for (int64_t rp = 0; rp < Ki; rp++) {
+          const double E1 = q->E_j[r1], Ep = q->E_i[rp];
+          chain_add(&gt, E1 - Ep, OP1(r1, rp) * OP2(r1, rp) * exp(-Ep / T) / Z, q->factor);
+        }
+    } else if (Kj > 0 && di > Ki) {
+      /* Eq. (15): op1_rho = prod_adj_fit_left(op1, rho_j) = op1[:n, :]^T rho_j, dim_i x n (numerics.hpp:308-316) */
+      const int64_t n = q->n_rho_j;
+      double *op1_rho = (double *)calloc((size_t)(di * n) + 1, sizeof(double));
+      gemm_rm(1, 0, di, n, n, 1.0, op1, ld, q->rho_j, n, 0.0, op1_rho, n);
+      for (int64_t rk = 0; rk < Kj; rk++)       /* kept in I1 x discarded in Ip */
+        for (int64_t rl = Ki; rl < di; rl++) {
+          const double Ek = q->E_j[rk], El = q->E_i[rl];
+          chain_add(&gt, Ek - El, op1_rho[rl * n + rk] * OP2(rk, rl), q->factor);
+        }
+      free(op1_rho);
+    }
+    /* Algo_CFSls::calc (c++/algo_CFS.hpp:29-56) */
+    if (last) {
+      for (int64_t r1 = 0; r1 < Kj; r1++)
+        for (int64_t rp = 0; rp < Ki; rp++) {
+          const double E1 = q->E_j[r1], Ep = q->E_i[rp];
+          chain_add(&ls, E1 - Ep, OP1(r1, rp) * OP2(r1, rp) * exp(-E1 / T) * (-sign) / Z, q->factor);
+        }
+    } else if (Ki > 0 && dj > Kj) {
+      /* Eq. (16): op2_rho = prod_fit_left(op2, rho_i) = op2[:, :n] rho_i, dim_j x n (numerics.hpp:283-290) */
+      int64_t n = 0;
+      double *op2_rho = op_times_rho(op2, ld, dj, di, q->rho_i, q->n_rho_i, &n);
+      for (int64_t rl = Kj; rl < dj; rl++)      /* discarded in I1 x kept in Ip */
+        for (int64_t rk = 0; rk < Ki; rk++) {
+          const double El = q->E_j[rl], Ek = q->E_i[rk];
+          chain_add(&ls, El - Ek, OP1(rl, rk) * op2_rho[rl * n + rk] * (-sign), q->factor);
+        }
+      free(op2_rho);
+    }
+#undef OP1
+#undef OP2
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * FT spectral weights (Algo_FT::calc, c++/algo_FT.hpp:28-44): stat_factor and the kept x kept loop.
+ * ------------------------------------------------------------------------- */
+int oracle_ft_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z, int sign,
+                  double *out_pos, double *out_neg) {
+  chain_t cb = {grid, out_pos, out_neg};
+  const double beta = 1.0 / T;
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair *q = &pairs[p];
+    const int64_t ld = q->ld_op;
+    for (int64_t r1 = 0; r1 < q->K_j; r1++)
+      for (int64_t rp = 0; rp < q->K_i; rp++) {
+        const double E1 = q->E_j[r1], Ep = q->E_i[rp];
+        const double stat = ((-sign) * exp(-beta * E1) + exp(-beta * Ep)) / Z;
+        chain_add(&cb, E1 - Ep, q->op1[r1 * ld + rp] * q->op2[r1 * ld + rp] * stat, q->factor);
+      }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Matsubara-axis Green's functions: Algo_FDMmats (c++/algo_FDM.hpp:167-213), Algo_DMNRGmats
+ * (c++/algo_DMNRG.hpp:74-100), Algo_FTmats (c++/algo_FT.hpp:59-84), each adding factor * term(n)
+ * for n < mats into G (ChainMatsubara::add), the frequencies ww(n) of c++/matsubara.hpp:20-27 and
+ * WEIGHT_TOL = 1e-8 (c++/numerics.hpp:318). G: (Re, Im) interleaved. The complex division is C's
+ * (the same routine std::complex<double> division reaches).
+ * ------------------------------------------------------------------------- */
+#define MATS_TOL 1e-8
+
+static double ww(int64_t n, int bosonic, double T) { return T * M_PI * (bosonic ? (2 * n) : (2 * n + 1)); }
+
+static void gadd(double *G, int64_t n, double complex z) {
+  G[2 * n] += creal(z);
+  G[2 * n + 1] += cimag(z);
+}
+
+/* Algo_FDMmats: term1 over D_i x D_j, term2 over D_i x K_j, term3 over K_i x D_j, in that order */
+static void mats_fdm_pair(const nrg_fdm_pair *q, int64_t mats, double T, double wnf, int sign, double *G) {
+  const int bos = sign == 1;
+  const int64_t di = q->dim_i, dj = q->dim_j, Ki = q->K_i, Kj = q->K_j, ld = q->ld_op;
+  const double *op1 = q->op1, *op2 = q->op2;
+  int64_t rows = 0, cols = 0;
+  double *rho_op2 = rho_times_op(q->rho_j, q->n_rho_j, op2, ld, dj, di, &rows);  /* prod_fit(rhoFDM.at(Ij), op2) */
+  double *op2_rho = op_times_rho(op2, ld, dj, di, q->rho_i, q->n_rho_i, &cols);  /* prod_fit(op2, rhoFDM.at(Ii)) */
+  for (int64_t i = Ki; i < di; i++)
+    for (int64_t j = Kj; j < dj; j++) {
+      const double Ei = q->E_i[i], Ej = q->E_j[j], energy = Ej - Ei;
+      const double o = op1[j * ld + i] * op2[j * ld + i];
+      const double wA = o * wnf * exp(-Ei / T), wB = o * (-sign) * wnf * exp(-Ej / T);
+      for (int64_t n = 0; n < mats; n++) {
+        double complex t;
+        if (!bos || n > 0 || fabs(energy) > MATS_TOL) t = (wA + wB) / (ww(n, bos, T) * I - energy);
+        else t = -wA / T;
+        gadd(G, n, t * q->factor);
+      }
+    }
+  for (int64_t i = Ki; i < di; i++)
+    for (int64_t j = 0; j < Kj; j++) {
+      const double Ei = q->E_i[i], Ej = q->E_j[j], energy = Ej - Ei;
+      const double wA = op1[j * ld + i] * op2[j * ld + i] * wnf * exp(-Ei / T);
+      const double wB = op1[j * ld + i] * rho_op2[j * di + i] * (-sign);
+      for (int64_t n = 0; n < mats; n++) gadd(G, n, (wA + wB) / (ww(n, bos, T) * I - energy) * q->factor);
+    }
+  for (int64_t i = 0; i < Ki; i++)
+    for (int64_t j = Kj; j < dj; j++) {
+      const double Ei = q->E_i[i], Ej = q->E_j[j], energy = Ej - Ei;
+      const double wA = op1[j * ld + i] * op2_rho[j * cols + i];
+      const double wB = (-sign) * op1[j * ld + i] * op2[j * ld + i] * wnf * exp(-Ej / T);
+      for (int64_t n = 0; n < mats; n++) gadd(G, n, (wA + wB) / (ww(n, bos, T) * I - energy) * q->factor);
+    }
+  free(rho_op2);
+  free(op2_rho);
+}
+
+/* Algo_DMNRGmats: the weights lambda of Algo_DMNRG, kept x kept */
+static void mats_dmnrg_pair(const nrg_fdm_pair *q, int64_t mats, double T, int sign, double *G) {
+  const int bos = sign == 1;
+  const int64_t ld = q->ld_op;
+  const double *op1 = q->op1, *op2 = q->op2;
+  for (int64_t rm = 0; rm < q->K_i; rm++)
+    for (int64_t rj = 0; rj < q->K_j; rj++) {
+      const double Em = q->E_i[rm], Ej = q->E_j[rj], energy = Ej - Em;
+      double sumA = 0.0, sumB = 0.0;
+      for (int64_t ri = 0; ri < q->K_i; ri++) sumA += op2[rj * ld + ri] * q->rho_i[rm * q->n_rho_i + ri];
+      const double wA = sumA * op1[rj * ld + rm];
+      for (int64_t ri = 0; ri < q->K_j; ri++) sumB += op1[ri * ld + rm] * q->rho_j[rj * q->n_rho_j + ri];
+      const double wB = sumB * op2[rj * ld + rm];
+      for (int64_t n = 0; n < mats; n++) {
+        double complex t;
+        if (!bos || n > 0 || fabs(energy) > MATS_TOL) t = (wA + (-sign) * wB) / (ww(n, bos, T) * I - energy);
+        else t = -wA / T;
+        gadd(G, n, q->factor * t);
+      }
+    }
+}
+
+/* Algo_FTmats: stat_factor over kept x kept */
+static void mats_ft_pair(const nrg_fdm_pair *q, int64_t mats, double T, double Z, int sign, double *G) {
+  const int bos = sign == 1;
+  const double beta = 1.0 / T;
+  const int64_t ld = q->ld_op;
+  for (int64_t r1 = 0; r1 < q->K_j; r1++)
+    for (int64_t rp = 0; rp < q->K_i; rp++) {
+      const double E1 = q->E_j[r1], Ep = q->E_i[rp], energy = E1 - Ep;
+      const double o = q->op1[r1 * ld + rp] * q->op2[r1 * ld + rp];
+      for (int64_t n = 0; n < mats; n++) {
+        double complex stat;
+        if (!bos || n > 0 || fabs(energy) > MATS_TOL)
+          stat = ((-sign) * exp(-beta * E1) + exp(-beta * Ep)) / (Z * (ww(n, bos, T) * I - energy));
+        else
+          stat = -exp(-beta * E1) / (Z * T);
+        gadd(G, n, q->factor * (o * stat));
+      }
+    }
+}
+
+/* mode 0: FDMmats, 1: DMNRGmats, 2: FTmats */
+int oracle_mats(int mode, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double wnf, double Z, int sign,
+                double *G) {
+  if (mode == 0 && !p_dgemm) return -1;
+  for (int p = 0; p < npairs; p++) {
+    if (mode == 0) mats_fdm_pair(&pairs[p], mats, T, wnf, sign, G);
+    else if (mode == 1) mats_dmnrg_pair(&pairs[p], mats, T, sign, G);
+    else mats_ft_pair(&pairs[p], mats, T, Z, sign, G);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Temperature-dependent sums: Algo_GT<n> (c++/algo_FT.hpp:95-134) and Algo_CHIT (136-170) with
+ * chit_weight (c++/numerics.hpp:323-336). kind 0..2: GT, I1T, I2T; 3: CHIT. Returns the sum over
+ * pairs of factor * value, value the pair's kept x kept sum in the reference's loop order.
+ * ------------------------------------------------------------------------------------------ */
+static double chit_weight_c(double En, double Em, double beta) {
+  const double betaEn = beta * En, betaEm = beta * Em, x = betaEn - betaEm;
+  if (fabs(x) > MATS_TOL) return (exp(-betaEm) - exp(-betaEn)) / x;
+  return exp(-betaEm);
+}
+
+double oracle_temp_sum(int kind, const nrg_fdm_pair *pairs, int npairs, double temperature, double Z) {
+  const double beta = 1.0 / temperature;
+  double total = 0.0;
+  for (int p = 0; p < npairs; p++) {
+    const nrg_fdm_pair *q = &pairs[p];
+    const int64_t ld = q->ld_op;
+    double value = 0.0;
+    for (int64_t r1 = 0; r1 < q->K_j; r1++)
+      for (int64_t rp = 0; rp < q->K_i; rp++) {
+        const double E1 = q->E_j[r1], Ep = q->E_i[rp];
+        double stat;
+        if (kind == 3) stat = chit_weight_c(E1, Ep, beta) / Z;
+        else stat = beta / (exp(+beta * E1) + exp(+beta * Ep)) * pow(E1 - Ep, kind) / Z;
+        value += q->op1[r1 * ld + rp] * q->op2[r1 * ld + rp] * stat;
+      }
+    total += q->factor * value;
+  }
+  return total;
 }
 
 /* ---------------------------------------------------------------------------

@@ -18,6 +18,13 @@ int oracle_fdm_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *gr
                    double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg);
 int oracle_dmnrg_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin, double emax, int sign,
                      double *out_pos, double *out_neg);
+int oracle_cfs_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z, int last, int sign,
+                   double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg);
+int oracle_ft_bin(const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double Z, int sign,
+                  double *out_pos, double *out_neg);
+int oracle_mats(int mode, const nrg_fdm_pair *pairs, int npairs, int64_t mats, double T, double wnf, double Z, int sign,
+                double *G);
+double oracle_temp_sum(int kind, const nrg_fdm_pair *pairs, int npairs, double temperature, double Z);
 void oracle_merge_nn2(const nrg_bin_grid *g, double *full, const double *cs, double Emin, double Ex, double Emax, int first,
                       int last, int zbw, double NNtanh, int nn2avg);
 int oracle_broaden(const double *pos_e, const double *pos_w, int64_t npos, const double *neg_e, const double *neg_w,
@@ -567,7 +574,7 @@ TEST(Spectrum, dmnrg_step_matches_oracle) {
     const auto &[Ij, Ii] = II;
     nrg_fdm_pair p{};
     p.dim_i = (int64_t)m.size2(); p.dim_j = (int64_t)m.size1();
-    p.K_i = (int64_t)diag.at(Ii).Krange().second; p.K_j = (int64_t)diag.at(Ij).Krange().second;
+    p.K_i = (int64_t)diag.at(Ii).getnrkept(); p.K_j = (int64_t)diag.at(Ij).getnrkept();   // kept()
     p.E_i = E[Ii].data(); p.E_j = E[Ij].data();
     p.op1 = m.data(); p.op2 = m.data(); p.ld_op = (int64_t)m.size2();
     p.rho_i = rho_h[Ii].data(); p.n_rho_i = (int64_t)rho_h[Ii].size1();
@@ -593,6 +600,264 @@ TEST(Spectrum, dmnrg_step_matches_oracle) {
   EXPECT_TRUE(err_f <= 1e-13 * scale_f);
   std::printf("    step %zu: %zu pairs, max bin deviation %.2e of %.3e (merged %.2e of %.3e)\n", step, pairs.size(), err_b,
               scale_b, err_f, scale_f);
+}
+
+TEST(Spectrum, cfs_steps_match_oracle) {
+  // a step below the last one (DM-NRG rho from the sweep) and the last step, accumulated as mergeCFS does
+  auto &c = chain();
+  Device &dev = Device::default_device();
+  const auto mult = c.sym.multfnc();
+  const size_t last = c.Nmax - 1;
+  DiagInfo dl = c.diags[last];
+  dl.truncate_perform();
+  DensMatElements rho = init_rho(1.0 / c.T * std::pow(2.0, -(double)last / 2), dl, mult);
+  std::vector<const DiagInfo *> dptr;
+  for (auto &d : c.diags) dptr.push_back(&d);
+  const size_t step = last - 1;   // a step with discarded states (24 kept per subspace)
+  calc_densitymatrix(rho, c.store, c.sym, dptr, step, c.Nmax, dev);
+  const BinGrid grid = BinGrid::standard(-1, -1, 1.0, std::pow(2.0, -(double)last / 2), 100, 1e-16);
+  const auto ff = [](const Invar &Ii, const Invar &) { return 1.0 + 0.25 * Ii.qn[0]; };
+  const double Z = 2.5;
+  CFSSpectrum spec(dev, grid, -1);
+  const nrg_bin_grid g = grid.c();
+  std::vector<double> tot(2 * grid.nbins, 0.0);
+  double err = 0, scale = 0;
+  for (const bool is_last : {false, true}) {
+    DiagInfo &diag = c.diags[is_last ? last : step];
+    MatrixElements op;
+    std::map<Twoinvar, Matrix> op_h;
+    for (const auto &[I, e] : diag) {
+      const Invar Ij{I.qn[0] + 1};
+      if (!diag.count(Ij)) continue;
+      Matrix m = random_matrix(diag.at(Ij).getnrall(), e.getnrall());
+      op_h[{Ij, I}] = m;
+      op[{Ij, I}] = DMatrix(m);
+    }
+    spec.calc(diag, op, op, rho, is_last, c.T, Z, ff);
+    std::vector<nrg_fdm_pair> pairs;
+    std::map<Invar, std::vector<double>> E;
+    std::map<Invar, Matrix> rho_h;
+    for (const auto &[I, e] : diag) E[I] = e.values.all_abs_zero();
+    for (auto &[I, m] : rho) rho_h[I] = m.host();
+    for (const auto &[II, m] : op_h) {
+      const auto &[Ij, Ii] = II;
+      nrg_fdm_pair p{};
+      p.dim_i = (int64_t)m.size2(); p.dim_j = (int64_t)m.size1();
+      p.K_i = (int64_t)diag.at(Ii).getnrkept(); p.K_j = (int64_t)diag.at(Ij).getnrkept();   // kept()
+      p.E_i = E[Ii].data(); p.E_j = E[Ij].data();
+      p.op1 = m.data(); p.op2 = m.data(); p.ld_op = (int64_t)m.size2();
+      if (!is_last) {
+        p.rho_i = rho_h[Ii].data(); p.n_rho_i = (int64_t)rho_h[Ii].size1();
+        p.rho_j = rho_h[Ij].data(); p.n_rho_j = (int64_t)rho_h[Ij].size1();
+      }
+      p.factor = ff(Ii, Ij);
+      pairs.push_back(p);
+    }
+    std::vector<double> b(4 * grid.nbins, 0.0);
+    const size_t nb = grid.nbins;
+    EXPECT_EQ(oracle_cfs_bin(pairs.data(), (int)pairs.size(), &g, c.T, Z, is_last ? 1 : 0, -1, b.data(), b.data() + nb,
+                             b.data() + 2 * nb, b.data() + 3 * nb),
+              0);
+    for (size_t i = 0; i < nb; i++) {
+      scale += std::abs(b[i]) + std::abs(b[nb + i]) + std::abs(b[2 * nb + i]) + std::abs(b[3 * nb + i]);
+      err = std::max({err, std::abs(spec.last_greater().pos[i] - b[i]), std::abs(spec.last_greater().neg[i] - b[nb + i]),
+                      std::abs(spec.last_lesser().pos[i] - b[2 * nb + i]),
+                      std::abs(spec.last_lesser().neg[i] - b[3 * nb + i])});
+      tot[i] += b[i] + b[2 * nb + i];
+      tot[nb + i] += b[nb + i] + b[3 * nb + i];
+    }
+    double w = 0;
+    for (double x : b) w += std::abs(x);
+    EXPECT_TRUE(w > 0);
+    std::printf("    %s step: %zu pairs, total |weight| %.3e\n", is_last ? "last" : "inner", pairs.size(), w);
+  }
+  double err_t = 0;
+  for (size_t i = 0; i < grid.nbins; i++)
+    err_t = std::max({err_t, std::abs(spec.total().pos[i] - tot[i]), std::abs(spec.total().neg[i] - tot[grid.nbins + i])});
+  EXPECT_TRUE(scale > 0);
+  EXPECT_TRUE(err <= 1e-13 * scale);
+  EXPECT_TRUE(err_t <= 2e-13 * scale);
+  std::printf("    max bin deviation %.2e (total %.2e) of %.3e\n", err, err_t, scale);
+}
+
+TEST(Spectrum, ft_steps_match_oracle) {
+  // an inner step and the last step (all 24 kept states of each subspace), bins and N/N+2 merge
+  auto &c = chain();
+  Device &dev = Device::default_device();
+  const size_t last = c.Nmax - 1;
+  const BinGrid grid = BinGrid::standard(-1, -1, 1.0, std::pow(2.0, -(double)last / 2), 100, 1e-16);
+  NN2Window win;
+  win.E0 = 1.5;
+  win.Efactor = 2.0;
+  const auto ff = [](const Invar &Ii, const Invar &) { return 1.0 - 0.125 * Ii.qn[0]; };
+  const double Z = 3.5;
+  FTSpectrum spec(dev, grid, -1, c.T, win);
+  const nrg_bin_grid g = grid.c();
+  const size_t nb = grid.nbins;
+  std::vector<double> fp(nb, 0.0), fn(nb, 0.0);
+  double err = 0, scale = 0;
+  for (const size_t step : {last - 1, last}) {
+    DiagInfo &diag = c.diags[step];
+    MatrixElements op;
+    std::map<Twoinvar, Matrix> op_h;
+    for (const auto &[I, e] : diag) {
+      const Invar Ij{I.qn[0] + 1};
+      if (!diag.count(Ij)) continue;
+      Matrix m = random_matrix(diag.at(Ij).getnrall(), e.getnrall());
+      op_h[{Ij, I}] = m;
+      op[{Ij, I}] = DMatrix(m);
+    }
+    const double sc = std::pow(2.0, -(double)step / 2);
+    const bool is_last = step == last;
+    spec.calc(diag, op, op, Z, sc, true, false, is_last, ff);
+    std::vector<nrg_fdm_pair> pairs;
+    std::map<Invar, std::vector<double>> E;
+    for (const auto &[I, e] : diag) E[I] = e.values.all_abs_zero();
+    for (const auto &[II, m] : op_h) {
+      const auto &[Ij, Ii] = II;
+      nrg_fdm_pair p{};
+      p.dim_i = (int64_t)m.size2(); p.dim_j = (int64_t)m.size1();
+      p.K_i = (int64_t)diag.at(Ii).getnrkept(); p.K_j = (int64_t)diag.at(Ij).getnrkept();
+      p.E_i = E[Ii].data(); p.E_j = E[Ij].data();
+      p.op1 = m.data(); p.op2 = m.data(); p.ld_op = (int64_t)m.size2();
+      p.factor = ff(Ii, Ij);
+      pairs.push_back(p);
+    }
+    std::vector<double> bp(nb, 0.0), bn(nb, 0.0);
+    EXPECT_EQ(oracle_ft_bin(pairs.data(), (int)pairs.size(), &g, c.T, Z, -1, bp.data(), bn.data()), 0);
+    oracle_merge_nn2(&g, fp.data(), bp.data(), win.emin(sc), win.ex(sc), win.emax(sc), 0, is_last, 0, 0.0, 0);
+    oracle_merge_nn2(&g, fn.data(), bn.data(), win.emin(sc), win.ex(sc), win.emax(sc), 0, is_last, 0, 0.0, 0);
+    double w = 0;
+    for (size_t i = 0; i < nb; i++) {
+      w += std::abs(bp[i]) + std::abs(bn[i]);
+      err = std::max({err, std::abs(spec.last_step().pos[i] - bp[i]), std::abs(spec.last_step().neg[i] - bn[i])});
+    }
+    EXPECT_TRUE(w > 0);
+    scale += w;
+    std::printf("    step %zu: %zu pairs, total |weight| %.3e\n", step, pairs.size(), w);
+  }
+  double err_m = 0;
+  for (size_t i = 0; i < nb; i++)
+    err_m = std::max({err_m, std::abs(spec.spectrum().pos[i] - fp[i]), std::abs(spec.spectrum().neg[i] - fn[i])});
+  EXPECT_TRUE(err <= 1e-13 * scale);
+  EXPECT_TRUE(err_m <= 1e-13 * scale);
+  std::printf("    max bin deviation %.2e (merged %.2e) of %.3e\n", err, err_m, scale);
+}
+
+TEST(Spectrum, matsubara_steps_match_oracle) {
+  // FDMmats at the step below the last (rho_FDM from the sweep), FTmats at the last step, both
+  // accumulated into one MatsubaraGF, as GFMatsubara::merge does across steps
+  auto &c = chain();
+  Device &dev = Device::default_device();
+  const auto mult = c.sym.multfnc();
+  const size_t N = c.Nmax - 1, step = N - 1;
+  DensMatElements rho = init_rho_FDM(N, c.store, c.stats, mult, c.T);
+  std::vector<const DiagInfo *> dptr;
+  for (auto &d : c.diags) dptr.push_back(&d);
+  calc_fulldensitymatrix(rho, c.store, c.stats, c.sym, dptr, step, c.Nmax, c.T, dev);
+  const size_t mats = 64;
+  const double Z = 2.0;
+  const auto ff = [](const Invar &Ii, const Invar &) { return 1.0 + 0.25 * Ii.qn[0]; };
+  MatsubaraGF gf(dev, mats, -1, c.T);
+  std::vector<double> want(2 * mats, 0.0);
+  double err = 0, scale = 0;
+  for (const bool ft : {false, true}) {
+    DiagInfo &diag = c.diags[ft ? N : step];
+    MatrixElements op;
+    std::map<Twoinvar, Matrix> op_h;
+    for (const auto &[I, e] : diag) {
+      const Invar Ij{I.qn[0] + 1};
+      if (!diag.count(Ij)) continue;
+      Matrix m = random_matrix(diag.at(Ij).getnrall(), e.getnrall());
+      op_h[{Ij, I}] = m;
+      op[{Ij, I}] = DMatrix(m);
+    }
+    if (ft) gf.calc_ft(diag, op, op, Z, ff);
+    else gf.calc_fdm(diag, op, op, rho, c.stats.wnfactor[step], ff);
+    std::vector<nrg_fdm_pair> pairs;
+    std::map<Invar, std::vector<double>> E;
+    std::map<Invar, Matrix> rho_h;
+    for (const auto &[I, e] : diag) E[I] = ft ? e.values.all_abs_zero() : e.values.all_abs_G();
+    for (auto &[I, m] : rho) rho_h[I] = m.host();
+    for (const auto &[II, m] : op_h) {
+      const auto &[Ij, Ii] = II;
+      nrg_fdm_pair p{};
+      p.dim_i = (int64_t)m.size2(); p.dim_j = (int64_t)m.size1();
+      p.K_i = (int64_t)(ft ? diag.at(Ii).getnrkept() : diag.at(Ii).Krange().second);
+      p.K_j = (int64_t)(ft ? diag.at(Ij).getnrkept() : diag.at(Ij).Krange().second);
+      p.E_i = E[Ii].data(); p.E_j = E[Ij].data();
+      p.op1 = m.data(); p.op2 = m.data(); p.ld_op = (int64_t)m.size2();
+      if (!ft && rho_h.count(Ii)) { p.rho_i = rho_h[Ii].data(); p.n_rho_i = (int64_t)rho_h[Ii].size1(); }
+      if (!ft && rho_h.count(Ij)) { p.rho_j = rho_h[Ij].data(); p.n_rho_j = (int64_t)rho_h[Ij].size1(); }
+      p.factor = ff(Ii, Ij);
+      pairs.push_back(p);
+    }
+    std::vector<double> G(2 * mats, 0.0);
+    EXPECT_EQ(oracle_mats(ft ? 2 : 0, pairs.data(), (int)pairs.size(), (int64_t)mats, c.T, c.stats.wnfactor[step], Z, -1,
+                          G.data()),
+              0);
+    for (size_t n = 0; n < mats; n++) {
+      const std::complex<double> o(G[2 * n], G[2 * n + 1]);
+      err = std::max(err, std::abs(gf.last_step()[n] - o));
+      scale = std::max(scale, std::abs(o));
+      want[2 * n] += G[2 * n];
+      want[2 * n + 1] += G[2 * n + 1];
+    }
+    std::printf("    %s: %zu pairs, |G(i w_0)| = %.3e\n", ft ? "FTmats (last step)" : "FDMmats", pairs.size(),
+                std::hypot(G[0], G[1]));
+  }
+  double err_t = 0;
+  for (size_t n = 0; n < mats; n++)
+    err_t = std::max(err_t, std::abs(gf.values()[n] - std::complex<double>(want[2 * n], want[2 * n + 1])));
+  EXPECT_TRUE(scale > 0);
+  EXPECT_TRUE(err <= 1e-12 * scale);
+  EXPECT_TRUE(err_t <= 2e-12 * scale);
+  std::printf("    max deviation %.2e (accumulated %.2e) of max |G| %.3e\n", err, err_t, scale);
+}
+
+TEST(Spectrum, temperature_series_match_oracle) {
+  // G(T) and chi(T) of two steps, at the temperatures gtp * scale(N)
+  auto &c = chain();
+  Device &dev = Device::default_device();
+  const size_t N = c.Nmax - 1;
+  const auto ff = [](const Invar &Ii, const Invar &) { return 1.0 + 0.5 * Ii.qn[0]; };
+  for (const int kind : {NRG_TEMP_GT, NRG_TEMP_CHIT}) {
+    TempSeries ts(dev, kind);
+    for (const size_t step : {N - 1, N}) {
+      DiagInfo &diag = c.diags[step];
+      MatrixElements op;
+      std::map<Twoinvar, Matrix> op_h;
+      for (const auto &[I, e] : diag) {
+        const Invar Ij{I.qn[0] + 1};
+        if (!diag.count(Ij)) continue;
+        Matrix m = random_matrix(diag.at(Ij).getnrall(), e.getnrall());
+        op_h[{Ij, I}] = m;
+        op[{Ij, I}] = DMatrix(m);
+      }
+      const double temperature = 0.4 * std::pow(2.0, -(double)step / 2), Z = 1.7;
+      const double got = ts.calc(diag, op, op, temperature, Z, ff);
+      std::vector<nrg_fdm_pair> pairs;
+      std::map<Invar, std::vector<double>> E;
+      for (const auto &[I, e] : diag) E[I] = e.values.all_abs_zero();
+      double scale = 0;
+      for (const auto &[II, m] : op_h) {
+        const auto &[Ij, Ii] = II;
+        nrg_fdm_pair p{};
+        p.dim_i = (int64_t)m.size2(); p.dim_j = (int64_t)m.size1();
+        p.K_i = (int64_t)diag.at(Ii).getnrkept(); p.K_j = (int64_t)diag.at(Ij).getnrkept();
+        p.E_i = E[Ii].data(); p.E_j = E[Ij].data();
+        p.op1 = m.data(); p.op2 = m.data(); p.ld_op = (int64_t)m.size2();
+        p.factor = ff(Ii, Ij);
+        pairs.push_back(p);
+        for (size_t r = 0; r < (size_t)p.K_j; r++)
+          for (size_t q = 0; q < (size_t)p.K_i; q++) scale = std::max(scale, std::abs(m(r, q) * m(r, q) * p.factor));
+      }
+      const double want = oracle_temp_sum(kind, pairs.data(), (int)pairs.size(), temperature, Z);
+      EXPECT_TRUE(std::abs(got - want) <= 1e-12 * std::max(std::abs(want), 1e-300) + 1e-15 * scale);
+      std::printf("    kind %d step %zu: %.6e (oracle %.6e)\n", kind, step, got, want);
+    }
+    EXPECT_EQ(ts.points().size(), (size_t)2);
+  }
 }
 
 TEST(Spectrum, continuous_matches_oracle) {
