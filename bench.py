@@ -653,7 +653,7 @@ def bench_mats(args):
             el += p["dim_i"] * p["dim_j"] - p["K_i"] * p["K_j"]
         oracle.lib().oracle_blas_threads(1)
         reps, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < 10.0 and reps < 100:
+        while time.perf_counter() - t0 < 10.0 and reps < 5000:
             oracle.mats("fdm", hs, mats, T, -1, wnf=wnf)
             reps += 1
         dt = time.perf_counter() - t0
