@@ -10,3 +10,4 @@
 #include "measurements.hpp"
 #include "dmnrg.hpp"
 #include "spectrum.hpp"
+#include "store.hpp"

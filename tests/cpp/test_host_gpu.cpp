@@ -860,6 +860,60 @@ TEST(Spectrum, temperature_series_match_oracle) {
   }
 }
 
+TEST(Store, steps_round_trip_through_the_budget) {
+  // every step's eigenvectors (with their U(i) blocks) and a density matrix per step, under a
+  // device budget of about two steps: the earliest move to pinned host memory and come back intact
+  auto &c = chain();
+  std::vector<DiagInfo> orig(c.Nmax);
+  size_t step_bytes = 0;
+  for (size_t N = 1; N < c.Nmax; N++) {
+    orig[N] = c.diags[N];
+    split_in_blocks(orig[N], c.sstruct[N]);
+    for (const auto &[I, e] : orig[N]) step_bytes = std::max(step_bytes, 8 * e.vectors.dim() * e.vectors.M());
+  }
+  StepStore store(3 * step_bytes);
+  std::map<size_t, std::map<Invar, Matrix>> rho_h;
+  for (size_t N = 1; N < c.Nmax; N++) {
+    store.save(N, orig[N]);
+    DensMatElements rho;   // any matrices of the kept sizes
+    for (const auto &[I, e] : c.diags[N]) {
+      rho_h[N][I] = random_matrix(e.getnrkept(), e.getnrkept());
+      rho.emplace(I, DMatrix(rho_h[N][I]));
+    }
+    store.save_rho(N, "rho", std::move(rho));
+    EXPECT_TRUE(store.device_bytes() <= store.budget());
+  }
+  EXPECT_TRUE(store.host_bytes() > 0);
+  std::printf("    device %zu B (budget %zu B), host %zu B\n", store.device_bytes(), store.budget(), store.host_bytes());
+  for (size_t N = c.Nmax - 1; N >= 1; N--) {   // the backward sweep's order
+    const DiagInfo d = store.load(N);
+    EXPECT_EQ(d.size(), orig[N].size());
+    for (const auto &[I, e] : orig[N]) {
+      const Eigen &g = d.at(I);
+      EXPECT_TRUE(g.values.all_rel() == e.values.all_rel());
+      EXPECT_TRUE(max_abs_diff(g.vectors.get(), e.vectors.get()) == 0.0);
+      EXPECT_EQ(g.U.size(), e.U.size());
+      for (size_t b = 0; b < e.U.size(); b++) {
+        const DView u = e.U.get(b), v = g.U.get(b);
+        EXPECT_TRUE(u.rows == v.rows && u.cols == v.cols);
+        if (!u.empty()) EXPECT_TRUE(max_abs_diff(download(u), download(v)) == 0.0);
+      }
+    }
+    const DensMatElements r = store.load_rho(N, "rho", true);
+    EXPECT_FALSE(store.has_rho(N, "rho"));
+    for (const auto &[I, m] : rho_h[N]) EXPECT_TRUE(max_abs_diff(r.at(I).host(), m) == 0.0);
+  }
+  (void)store.load(2, true);
+  EXPECT_FALSE(store.has(2));
+  bool threw = false;
+  try {
+    (void)store.load(2);
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+}
+
 TEST(Spectrum, continuous_matches_oracle) {
   // the FDM spectrum of the last step of the toy chain, trimmed and broadened as the reference saves it
   auto &c = chain();
