@@ -11,3 +11,4 @@
 #include "dmnrg.hpp"
 #include "spectrum.hpp"
 #include "store.hpp"
+#include "read_input.hpp"

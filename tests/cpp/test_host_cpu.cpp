@@ -190,7 +190,53 @@ static int thermo(const char *path) {
   return 0;
 }
 
+// --data IN OUT: read_input(IN), write_input to OUT, read OUT back and require the same data
+static bool same(const IrredMatrices &a, const IrredMatrices &b) {
+  if (a.size() != b.size()) return false;
+  for (const auto &[k, m] : a) {
+    const auto it = b.find(k);
+    if (it == b.end() || it->second.size1() != m.size1() || it->second.size2() != m.size2()) return false;
+    for (size_t i = 0; i < m.size1(); i++)
+      for (size_t j = 0; j < m.size2(); j++)
+        if (m(i, j) != it->second(i, j)) return false;
+  }
+  return true;
+}
+
+static int data_round_trip(const char *in, const char *out) {
+  const InputData d = read_input_file(in);
+  {
+    std::ofstream o(out);
+    write_input(o, d);
+  }
+  const InputData e = read_input_file(out);
+  bool ok = d.symtype == e.symtype && d.channels == e.channels && d.Nmax == e.Nmax && d.nsubs == e.nsubs &&
+            d.subspaces.size() == d.nsubs && d.subspaces == e.subspaces && d.GS_energy == e.GS_energy &&
+            d.has_GS_energy == e.has_GS_energy && d.coef == e.coef && d.coef_blocks == e.coef_blocks &&
+            d.opch.size() == e.opch.size() && d.ops.size() == e.ops.size();
+  for (size_t c = 0; ok && c < d.opch.size(); c++) {
+    ok = d.opch[c].size() == e.opch[c].size();
+    for (size_t j = 0; ok && j < d.opch[c].size(); j++) ok = same(d.opch[c][j], e.opch[c][j]);
+  }
+  for (size_t k = 0; ok && k < d.ops.size(); k++)
+    ok = d.ops[k].first == e.ops[k].first && d.ops[k].second.first == e.ops[k].second.first &&
+         same(d.ops[k].second.second, e.ops[k].second.second);
+  size_t nstates = 0;
+  for (const auto &[I, en] : d.subspaces) nstates += en.size();
+  std::printf("%s %zu %zu %zu %zu %zu %zu %zu %s\n", d.symtype.c_str(), d.channels, d.Nmax, d.nsubs, nstates,
+              d.opch.empty() ? 0 : d.opch[0].size(), d.ops.size(), d.coef.size(), ok ? "same" : "DIFFERENT");
+  return ok ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
+  if (argc == 4 && !std::strcmp(argv[1], "--data")) {
+    try {
+      return data_round_trip(argv[2], argv[3]);
+    } catch (const std::exception &e) {
+      std::printf("error: %s\n", e.what());
+      return 2;
+    }
+  }
   if (argc == 3 && !std::strcmp(argv[1], "--thermo")) {
     try {
       return thermo(argv[2]);

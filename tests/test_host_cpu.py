@@ -1,6 +1,7 @@
 """CPU tests of the C++ host layer (nrgljubljana_amd/host/nrg): the container/helper unit tests of
-tests/cpp/test_host_cpu.cpp, and calc_ZnD / fdm_thermodynamics (c++/measurements.hpp:59-163)
-against exact rational arithmetic.
+tests/cpp/test_host_cpu.cpp, calc_ZnD / fdm_thermodynamics (c++/measurements.hpp:59-163) against
+exact rational arithmetic, and the data-file reader (c++/read-input.hpp:23-135) on data files of the
+reference's own tests (tests/golden/data).
 
 The reference sums Z_n^D, Z_ZZG and the energy moments in 400-bit GMP floats and converts back with
 mpf_get_d (truncation toward zero). Exact rationals over the same double summands stand in for the
@@ -139,3 +140,42 @@ def test_thermo_weights_match_exact_arithmetic(tmp_path, seed, T):
         assert ulps(got[k], exp[k]) <= 1, (k, got[k], exp[k])
     assert ulps(got["C_fdm"], exp["C_fdm"]) <= 1, (got["C_fdm"], exp["C_fdm"])
     assert abs(sum(got[N][1] for N in range(st["Nb"], st["Ne"])) - 1.0) < 1e-12   # sum rule of w_n
+
+
+DATA = os.path.join(ROOT, "tests", "golden", "data")
+
+
+def numeric_tokens(path):
+    """Every number of a data file outside its '#' lines, in file order."""
+    out = []
+    with open(path) as f:
+        for line in f:
+            if line.lstrip().startswith("#"):
+                continue
+            for tok in line.split():
+                try:
+                    out.append(float(tok))
+                except ValueError:
+                    pass
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(f[:-4] for f in os.listdir(DATA) if f.endswith(".dat")))
+def test_data_file_reader(tmp_path, name):
+    """read_input -> write_input -> read_input gives the same data, and the rewritten file holds the
+    same numbers as the original (count, exact sum and sum of squares): nothing skipped or misread."""
+    src = os.path.join(DATA, name + ".dat")
+    dst = tmp_path / "data"
+    r = run("--data", src, str(dst))
+    assert r.returncode == 0, r.stdout + r.stderr
+    sym, channels, nmax, nsubs, nstates, perch, nops, ncoef, verdict = r.stdout.split()
+    assert verdict == "same"
+    with open(src) as f:
+        text = f.read()
+    assert sym == next(l.split()[-1] for l in text.splitlines() if "symtype" in l)
+    head = next(l for l in text.splitlines() if l.strip() and not l.startswith("#")).split()
+    assert (int(channels), int(nmax), int(nsubs)) == tuple(int(x) for x in head)
+    a, b = numeric_tokens(src), numeric_tokens(dst)
+    assert len(a) == len(b)
+    assert math.fsum(a) == math.fsum(b) and math.fsum(x * x for x in a) == math.fsum(x * x for x in b)
+    assert int(ncoef) >= 2 and int(nstates) >= int(nsubs)
