@@ -40,7 +40,7 @@ def main_panels(n, nb=32, tail=TAIL):
     return -(-(n - tail) // nb) if n > tail else 0
 
 
-def reduction_bytes(sizes, nb=32):
+def reduction_bytes(sizes, nb=32, groups=1):
     """Algorithmic HBM bytes of the two per-column kernels of the blocked Householder reduction
     (LAPACK dsytrd/dlatrd, lower), summed over the launches of one call:
 
@@ -53,24 +53,35 @@ def reduction_bytes(sizes, nb=32):
         previous column read, the column read and written and W's new column written:
         8 (2 jl r + ceil(r / 64) (r + 64) + 3 r)
 
-    Returns {kernel: (total bytes, launches)}; launch counts follow the library's schedule (one launch
-    per (panel, column) over all blocks still in their column-kernel panels)."""
+    Returns {kernel: (total bytes, launches)}; launch counts follow the library's schedule: the blocks
+    are dealt round-robin, largest first, into `groups` groups (the library's concurrent column
+    chains), and each group has one launch per (panel, column) while any of its blocks is still in
+    its column-kernel panels."""
     sizes = [int(n) for n in sizes]
+    order = sorted(range(len(sizes)), key=lambda b: -sizes[b])
+    grp = {b: k % groups for k, b in enumerate(order)}
     pmax = max(main_panels(n, nb) for n in sizes)
     prep_b = symv_b = 0.0
     prep_l = symv_l = 0
     for p in range(pmax):
-        ms = [n - p * nb for n in sizes if main_panels(n, nb) > p]
+        act = [b for b in range(len(sizes)) if main_panels(sizes[b], nb) > p]
+        ng = len({grp[b] for b in act})
         for jl in range(nb + 1):
-            for m in ms:
+            for b in act:
+                m = sizes[b] - p * nb
                 r = m - jl
                 prep_b += 8.0 * (2 * jl * r + ((r + 63) // 64) * (r + 64) + 3 * r)
                 if jl < nb:
                     t = m - jl - 1
                     symv_b += 8.0 * (t * (t + 1) / 2.0 + 2 * jl * t + 2 * t)
-            prep_l += 1
-            symv_l += jl < nb
+            prep_l += ng
+            symv_l += ng if jl < nb else 0
     return {"k_col_prep": (prep_b, prep_l), "k_col_symv": (symv_b, symv_l)}
+
+
+def red_groups():
+    """Concurrent column chains of the library's reduction: 2, or NRG_RED_STREAMS (1 .. 4)."""
+    return max(1, min(int(os.environ.get("NRG_RED_STREAMS", "2")), 4))
 
 
 def measured_traffic(kernel, args):
@@ -158,6 +169,8 @@ def bench_eig(args):
     elapsed = time.perf_counter() - t0
     ms_step = 1e3 * elapsed / args.steps
     syev_ms = ctx.last_time_ms(0)
+    from nrgljubljana_amd.core import phase_name
+    phases = {phase_name(k): round(ctx.last_time_ms(k), 3) for k in range(1, 5) if ctx.last_time_ms(k) > 0}
 
     # the same call with the input coming from, and the eigenpairs going back to, pinned host memory
     # (what a caller holding host matrices pays; not part of `value`)
@@ -185,7 +198,7 @@ def bench_eig(args):
         log(f"  {k:14s} {tms / 2:9.3f} ms/step  {cnt // 2:5d} launches/step")
     dom = max(kt.items(), key=lambda x: x[1][0])[0] if kt else None
     roofline = None
-    rb = reduction_bytes(sizes)
+    rb = reduction_bytes(sizes, groups=red_groups())
     if dom in rb:
         tot_bytes, nl = rb[dom]
         tms, cnt = kt[dom]
@@ -228,6 +241,7 @@ def bench_eig(args):
                    "form": args.mode, "blocks": int(len(sizes)), "sum_n": int(sizes.sum()), "max_n": int(sizes.max()),
                    "flops_per_step": flops, "alg_bytes_per_step": abytes},
         "syev_call_ms": round(syev_ms, 3),
+        "phases_ms": phases,   # of the last timed call, from HIP events on the library's stream
         "pcie_inclusive_ms_per_step": round(pcie_ms, 3),
         "frac_fp64_peak": round(gflops / 1e3 / FP64_PEAK_TFLOPS, 4),
         "max_eig_err_rel": eig_err,

@@ -110,6 +110,16 @@ void KTimer::release() {
   reset();
 }
 
+int ensure_aux_streams(nrg_ctx *ctx) {
+  if (ctx->fork_ev) return NRG_OK;
+  for (int i = 0; i < nrg_ctx::NAUX; i++) {
+    if (!ctx->aux[i]) NRG_HIP(hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
+    if (!ctx->join_ev[i]) NRG_HIP(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+  }
+  NRG_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+  return NRG_OK;
+}
+
 }  // namespace nrg
 
 using namespace nrg;
@@ -169,6 +179,11 @@ int nrg_ctx_destroy(nrg_ctx *c) {
   c->hflags.release();
   c->kt.release();
   for (int i = 0; i <= NPHASE; i++) hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < nrg_ctx::NAUX; i++) {
+    if (c->aux[i]) hipStreamDestroy(c->aux[i]);
+    if (c->join_ev[i]) hipEventDestroy(c->join_ev[i]);
+  }
+  if (c->fork_ev) hipEventDestroy(c->fork_ev);
   delete c;
   return NRG_OK;
 }

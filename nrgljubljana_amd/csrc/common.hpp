@@ -121,9 +121,16 @@ struct nrg_ctx {
   nrg::PinnedBuf hdesc;  // pinned staging of descriptor tables
   nrg::PinnedBuf hflags;
   nrg::KTimer kt;
+  // extra streams for independent work of one call (default priority, created on first use),
+  // forked from and joined back into the caller's stream with the events
+  static constexpr int NAUX = 3;
+  hipStream_t aux[NAUX] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {};
 };
 
 namespace nrg {
+// creates ctx->aux / fork_ev / join_ev once
+int ensure_aux_streams(nrg_ctx *ctx);
 int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
              double *w, int mode, int *info, hipStream_t st);
 int gemm3_run(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, int precision, hipStream_t st);

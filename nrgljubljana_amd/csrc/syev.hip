@@ -36,6 +36,7 @@ namespace nrg {
 constexpr int NB = 32;        // panel width of the Householder reduction
 constexpr int NT_PANEL = 1024; // threads of the column-preparation kernel
 constexpr int TAIL = 128;      // trailing order reduced inside one workgroup's LDS (k_tail)
+constexpr int RED_GROUPS = 2;  // block groups whose column chains run concurrently, one stream each
 constexpr int SMALLM = 64;    // merges up to this order do their own block-diagonal product
 // work-list entry of the column kernels: block index in the low bits, its panel above them
 constexpr int COL_PANEL_SHIFT = 20;
@@ -1501,6 +1502,17 @@ __global__ __launch_bounds__(256) void k_ref_copy(RefArgs r, double *H, const in
   for (size_t i = beg + threadIdx.x; i < end; i += 256) H[bd.offH + i] = r.X[bd.offZ + i];
 }
 
+// Concurrent column chains of the reduction: RED_GROUPS, or NRG_RED_STREAMS (1 .. 1 + NAUX) for
+// measurements
+static int red_groups() {
+  static const int g = [] {
+    const char *e = std::getenv("NRG_RED_STREAMS");
+    const int v = e ? std::atoi(e) : RED_GROUPS;
+    return std::max(1, std::min(v, 1 + nrg_ctx::NAUX));
+  }();
+  return g;
+}
+
 // ==========================================================================================
 // host driver
 // ==========================================================================================
@@ -1520,6 +1532,8 @@ int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, dou
       set_error("nrg_syev_batched: block order out of range [1, 4096]");
       return NRG_E_ARG;
     }
+  if (red_groups() > 1)
+    if (int r = ensure_aux_streams(ctx)) return r;
   if (mode == NRG_SYEV_MIXED) {
     extern int syev_mixed_run(nrg_ctx *, int, const int64_t *, const int64_t *, double *, const int64_t *,
                               double *, int *, hipStream_t);
@@ -1540,7 +1554,10 @@ struct SyevPlan {
   // work lists, all packed into one int array; offsets in ints
   std::vector<int> ibuf;
   size_t symm_off = 0, symm_cnt = 0;
-  struct ColLaunch { int p, jl, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };   // p: panel slot
+  // reduced column jl of panel slot p for the blocks of group g
+  struct ColLaunch { int p, jl, g, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };
+  int ngrp = 1;
+  std::vector<int> grp;   // group of every block
   std::vector<ColLaunch> cols;
   int64_t sumYp = 0;
   size_t all_off = 0;  // list 0..nblk-1
@@ -1558,7 +1575,7 @@ struct SyevPlan {
   size_t ym_off = 0, ym_cnt = 0, tb_off = 0, tb_cnt = 0;
   size_t g_items = 0, g_cnt = 0, yh_items = 0, yh_cnt = 0;
   std::vector<size_t> w_items, w_cnt, u_items, u_cnt;
-  std::vector<size_t> s2_items, s2_cnt;   // FP64 trailing rank-2k updates per panel
+  std::vector<size_t> s2_items, s2_cnt;   // trailing rank-2k updates per (panel slot, group): [q * ngrp + g]
 };
 
 // Panels reduced by the column kernels: those that start with more than TAIL trailing rows. The
@@ -1568,7 +1585,8 @@ static int main_panels(int64_t n) {
 }
 
 static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *off_H, const int64_t *off_w,
-                       bool fp64_inplace, int elem_bytes) {
+                       bool fp64_inplace, int elem_bytes, int ngrp) {
+  pl.ngrp = std::max(1, std::min(ngrp, nblk));
   pl.bd.resize(nblk);
   int64_t zo = 0, vo = 0, wo = 0, ao = 0;
   for (int b = 0; b < nblk; b++) {
@@ -1607,7 +1625,13 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   pl.symm_cnt = (I.size() - pl.symm_off) / 2;
   // reduced columns (two launches each) and trailing updates, on a common sequence of Pmax panel
   // slots: block b works on panel panel_in_slot(q, P_b) in slot q. Work-list entries of the column
-  // kernels carry the block and its panel (col_entry).
+  // kernels carry the block and its panel (col_entry). The blocks are dealt round-robin, largest
+  // first, into ngrp groups whose chains of column launches are independent: each group runs on a
+  // stream of its own, so one group's latency-bound column preparation overlaps another group's
+  // bandwidth-bound symmetric product. Launches are listed slot by slot, column by column, group by
+  // group, the order in which they are issued.
+  pl.grp.assign(nblk, 0);
+  for (size_t k = 0; k < order.size(); k++) pl.grp[order[k]] = (int)(k % pl.ngrp);
   pl.cols.clear();
   for (int q = 0; q < pl.Pmax; q++) {
     auto panel = [&](int b) { return panel_in_slot(q, pl.bd[b].P); };
@@ -1617,32 +1641,35 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     };
     int wmax = 0;
     for (int b : order) if (panel(b) >= 0) wmax = std::max(wmax, width(b));
-    for (int jl = 0; jl <= wmax; jl++) {
-      SyevPlan::ColLaunch c;
-      c.p = q; c.jl = jl;
-      c.prep_off = I.size();
-      c.rows = 0;
-      for (int b : order)
-        if (panel(b) >= 0 && jl <= width(b)) {
-          I.push_back(col_entry(b, panel(b)));
-          c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * panel(b) - jl));
+    for (int jl = 0; jl <= wmax; jl++)
+      for (int g = 0; g < pl.ngrp; g++) {
+        SyevPlan::ColLaunch c;
+        c.p = q; c.jl = jl; c.g = g;
+        c.prep_off = I.size();
+        c.rows = 0;
+        for (int b : order)
+          if (pl.grp[b] == g && panel(b) >= 0 && jl <= width(b)) {
+            I.push_back(col_entry(b, panel(b)));
+            c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * panel(b) - jl));
+          }
+        c.prep_cnt = I.size() - c.prep_off;
+        if (!c.prep_cnt) continue;   // every symv block is a prep block too
+        c.symv_off = I.size();
+        for (int b : order)
+          if (pl.grp[b] == g && panel(b) >= 0 && jl < width(b)) I.push_back(col_entry(b, panel(b)));
+        c.nact = I.size() - c.symv_off;
+        c.tpre_off = I.size();
+        int64_t tiles = 0;
+        for (size_t k = 0; k < c.nact; k++) {
+          const int b = I[c.symv_off + k] & COL_BLK_MASK;
+          const int64_t t = n[b] - (int64_t)NB * panel(b) - jl - 1;
+          const int64_t T = (t + 63) / 64;
+          I.push_back((int)tiles);
+          tiles += T * (T + 1) / 2 + (jl > 0 ? T : 0);   // + one dot-product workgroup per segment
         }
-      c.prep_cnt = I.size() - c.prep_off;
-      c.symv_off = I.size();
-      for (int b : order) if (panel(b) >= 0 && jl < width(b)) I.push_back(col_entry(b, panel(b)));
-      c.nact = I.size() - c.symv_off;
-      c.tpre_off = I.size();
-      int64_t tiles = 0;
-      for (size_t k = 0; k < c.nact; k++) {
-        const int b = I[c.symv_off + k] & COL_BLK_MASK;
-        const int64_t t = n[b] - (int64_t)NB * panel(b) - jl - 1;
-        const int64_t T = (t + 63) / 64;
-        I.push_back((int)tiles);
-        tiles += T * (T + 1) / 2 + (jl > 0 ? T : 0);   // + one dot-product workgroup per segment
+        c.tiles = (size_t)tiles;
+        pl.cols.push_back(c);
       }
-      c.tiles = (size_t)tiles;
-      pl.cols.push_back(c);
-    }
   }
   // tail of every block (one k_tail launch after the column launches)
   pl.tail_off = I.size();
@@ -1759,10 +1786,13 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
   gd.clear();
   std::vector<int> &I = pl.ibuf;
   // trailing updates A22 -= V W^T + W V^T after every panel that is not a block's last
-  pl.s2_items.assign(pl.Pmax, 0); pl.s2_cnt.assign(pl.Pmax, 0);
-  for (int q = 0; q < pl.Pmax; q++) {   // panel slot q, as in build_plan
+  pl.s2_items.assign((size_t)pl.Pmax * pl.ngrp, 0); pl.s2_cnt.assign((size_t)pl.Pmax * pl.ngrp, 0);
+  for (int q = 0; q < pl.Pmax; q++)   // panel slot q, as in build_plan
+  for (int gr = 0; gr < pl.ngrp; gr++) {
     const size_t d0 = gd.size();
-    for (const BlkDesc &d : pl.bd) {
+    for (size_t b = 0; b < pl.bd.size(); b++) {
+      const BlkDesc &d = pl.bd[b];
+      if (pl.grp[b] != gr) continue;
       const int p = panel_in_slot(q, d.P);
       if (p < 0) continue;
       const int n = (int)d.n, i0 = p * NB, mt = n - i0 - NB;
@@ -1777,7 +1807,8 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
     }
     // only the tiles on or below the first tile super-diagonal: the reduction reads the lower
     // triangle, and the symmetric product's diagonal 64x64 tiles reach at most 63 columns above it
-    pl.s2_items[q] = I.size();
+    const size_t qg = (size_t)q * pl.ngrp + gr;
+    pl.s2_items[qg] = I.size();
     for (size_t i = d0; i < gd.size(); i++) {
       const int tm = (gd[i].M + GEMM_TILE - 1) / GEMM_TILE;
       const int tiles = tm * ((gd[i].N + GEMM_TILE - 1) / GEMM_TILE);
@@ -1787,7 +1818,7 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
           I.push_back(k);
         }
     }
-    pl.s2_cnt[q] = (I.size() - pl.s2_items[q]) / 2;
+    pl.s2_cnt[qg] = (I.size() - pl.s2_items[qg]) / 2;
   }
   // back-transformation products
   const int nblk = (int)pl.bd.size();
@@ -1892,25 +1923,39 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     NRG_TIMED(ctx, st, "k_absmax", hipLaunchKernelGGL(k_absmax<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off)));
   if (pl.symm_cnt)
     NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off), 0));
-  // 2. reduction
+  // 2. reduction: group g's chain on stream S[g], forked from and joined back into st
+  const int NG = pl.ngrp;
+  hipStream_t S[1 + nrg_ctx::NAUX] = {st};
+  for (int g = 1; g < NG; g++) S[g] = ctx->aux[g - 1];
+  if (NG > 1) {
+    NRG_HIP(hipEventRecord(ctx->fork_ev, st));
+    for (int g = 1; g < NG; g++) NRG_HIP(hipStreamWaitEvent(S[g], ctx->fork_ev, 0));
+  }
   size_t ci = 0;
   for (int q = 0; q < pl.Pmax; q++) {
     for (; ci < pl.cols.size() && pl.cols[ci].p == q; ci++) {
       const SyevPlan::ColLaunch &c = pl.cols[ci];
-      if (c.prep_cnt) {
-        if (c.rows <= NT_PANEL)
-          NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, st, aT,
-                                                              I + c.prep_off, c.jl));
-        else
-          NRG_TIMED(ctx, st, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, NMAX / NT_PANEL>), grid(c.prep_cnt), dim3(NT_PANEL),
-                                                              0, st, aT, I + c.prep_off, c.jl));
-      }
+      hipStream_t sg = S[c.g];
+      if (c.rows <= NT_PANEL)
+        NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, sg, aT,
+                                                            I + c.prep_off, c.jl));
+      else
+        NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, NMAX / NT_PANEL>), grid(c.prep_cnt), dim3(NT_PANEL),
+                                                            0, sg, aT, I + c.prep_off, c.jl));
       if (c.tiles)
-        NRG_TIMED(ctx, st, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, st, aT,
+        NRG_TIMED(ctx, sg, "k_col_symv", hipLaunchKernelGGL(k_col_symv<T>, grid(c.tiles), dim3(256), 0, sg, aT,
                                                             I + c.symv_off, I + c.tpre_off, (int)c.nact, c.jl));
     }
-    launch_gemm<T>(ctx, st, L.dGT, I + pl.s2_items[q], H + pl.s2_items[q], pl.s2_cnt[q]);
+    for (int g = 0; g < NG; g++) {
+      const size_t qg = (size_t)q * NG + g;
+      launch_gemm<T>(ctx, S[g], L.dGT, I + pl.s2_items[qg], H + pl.s2_items[qg], pl.s2_cnt[qg]);
+    }
   }
+  if (NG > 1)
+    for (int g = 1; g < NG; g++) {
+      NRG_HIP(hipEventRecord(ctx->join_ev[g - 1], S[g]));
+      NRG_HIP(hipStreamWaitEvent(st, ctx->join_ev[g - 1], 0));
+    }
   if (pl.tail_cnt)
     NRG_TIMED(ctx, st, "k_tail", hipLaunchKernelGGL(k_tail<T>, grid(pl.tail_cnt), dim3(NT_PANEL), 0, st, aT, I + pl.tail_off));
   NRG_HIP(hipGetLastError());
@@ -1998,7 +2043,7 @@ static int finish_flags(nrg_ctx *ctx, int nblk, int *info, hipStream_t st) {
 int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                   double *w, int *info, hipStream_t st) {
   SyevPlan pl;
-  build_plan(pl, nblk, n, off_H, off_w, true, 8);
+  build_plan(pl, nblk, n, off_H, off_w, true, 8, red_groups());
   Carver cv;
   const size_t oZ0 = cv.take<double>(pl.sumn2), oZ1 = cv.take<double>(pl.sumn2), oS = cv.take<double>(pl.sumn2);
   const size_t oWp = cv.take<double>(pl.sumWp), oTau = cv.take<double>(pl.sumn);
@@ -2087,7 +2132,7 @@ static int refine_sweeps() {
 int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                    double *w, int *info, hipStream_t st) {
   SyevPlan pl;
-  build_plan(pl, nblk, n, off_H, off_w, false, 4);
+  build_plan(pl, nblk, n, off_H, off_w, false, 4, red_groups());
   // per-block chunk items for the elementwise refinement kernels
   const size_t rc_off = pl.ibuf.size();
   for (int b = 0; b < nblk; b++)
