@@ -153,8 +153,9 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items, 
 // 2a. column preparation (dlatrd, lower): one workgroup per block and reduced column.
 //     Finishes the W column of the previous reflector from the partials of k_col_symv, applies
 //     the panel's earlier reflectors to column jl and generates reflector jl. A thread owns the
-//     rows tid + q*NT_PANEL of the trailing part and issues every load of a row together, so a
-//     column costs a few memory round trips instead of one per dot product.
+//     rows tid + q*NT_PANEL of the trailing part and issues every load of a row together (with one
+//     row per thread, the first sixteen panel columns of the update too), so a column costs a few
+//     memory round trips instead of one per dot product.
 // ------------------------------------------------------------------------------------------
 constexpr int KCH = 16;   // panel columns loaded per batch
 
@@ -219,6 +220,22 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
       if (do_col) c[q] = A[(size_t)jl * n + r];
     }
   }
+  // the first KP panel columns of the update, loaded with the loads above (none when a thread owns
+  // several rows: registers). KP and NB are multiples of KC, so the batches below index the panel
+  // arrays cwv, wrow, vrow below NB (columns past jp carry zeros)
+  constexpr int KP = RPT > 1 ? 0 : KC;
+  static_assert(KP % KC == 0 && NB % KC == 0, "panel batches must tile the panel");
+  T ap[RPT][KP > 0 ? KP : 1], wp[RPT][KP > 0 ? KP : 1];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = jl + tid + q * NT_PANEL;
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      const bool in = r < m && k < jp;
+      ap[q][k] = in ? A[(size_t)k * n + r] : (T)0;
+      wp[q][k] = in ? W[(size_t)k * n + r] : (T)0;
+    }
+  }
   if (jl > 0) {
     __syncthreads();
     if (tid < 2 * NB) {
@@ -236,7 +253,12 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
       const int r = jl + tid + q * NT_PANEL;
       if (r < m) {
         T ay = 0, ac = 0;
-        for (int k0 = 0; k0 < jp; k0 += KC) {
+#pragma unroll
+        for (int k = 0; k < KP; k++) {
+          ay += ap[q][k] * cwv[k] + wp[q][k] * cwv[NB + k];
+          ac += ap[q][k] * wrow[k] + wp[q][k] * vrow[k];
+        }
+        for (int k0 = KP; k0 < jp; k0 += KC) {
           T ak[KC], wk[KC];
 #pragma unroll
           for (int k = 0; k < KC; k++) {
