@@ -4,6 +4,7 @@
 // fdm_thermodynamics results as hex floats for the exact-arithmetic check in Python.
 #include <cstdio>
 #include <fstream>
+#include <sstream>
 
 #include "minitest.hpp"
 #include "nrg/nrg.hpp"
@@ -121,6 +122,31 @@ TEST(Subspaces, dimensions_and_parts) {
   EXPECT_EQ(sd.part(2).second, 5u);
   EXPECT_FALSE(sd.exists(1));
   EXPECT_EQ(sd.total(), 5u);
+}
+
+// The data-file reader's error paths (c++/read-input.hpp: my_assert on the version, "Corrupted input
+// file." for an unknown subspace, std::invalid_argument for an unknown block)
+static InputData parse(const std::string &text) {
+  std::istringstream in(text);
+  return read_input(in);
+}
+TEST(ReadInput, header_blocks_and_errors) {
+  const std::string head = "# symtype QS\n#!9\n1 2 2\n0 1\n1\n0.\n1 2\n2\n0.5 1.5\nf 0 0\n1\n1 2 0 1\n0.25\n-0.5\n";
+  const InputData d = parse(head + "e\n-0.125\nsn_d\n1\n0 1 0 1\n2.\nz\n2\n1\n0.5\n0.25\n2\n0\n0\n0\n");
+  EXPECT_EQ(d.symtype, std::string("QS"));
+  EXPECT_EQ(d.nsubs, (size_t)2);
+  EXPECT_EQ(d.subspaces[1].second.size(), (size_t)2);
+  EXPECT_EQ(d.opch.size(), (size_t)1);
+  const Matrix &f = d.opch[0][0].at({Invar{1, 2}, Invar{0, 1}});
+  EXPECT_TRUE(f.size1() == 2 && f.size2() == 1 && f(0, 0) == 0.25 && f(1, 0) == -0.5);
+  EXPECT_TRUE(d.has_GS_energy && d.GS_energy == -0.125);
+  EXPECT_TRUE(d.ops.size() == 1 && d.ops[0].first == 's' && d.ops[0].second.first == "n_d");
+  EXPECT_TRUE(d.coef.at("xi")[0].size() == 3 && d.coef.at("zeta")[0][2] == 0.0);
+  EXPECT_THROW(parse("# symtype QS\n#!8\n1 2 2\n"));                       // version
+  EXPECT_THROW(parse("# symtype XYZ\n#!9\n1 2 2\n"));                      // symmetry type
+  EXPECT_THROW(parse(head + "w\n"));                                         // unknown block
+  EXPECT_THROW(parse("# symtype QS\n#!9\n1 2 1\n0 1\n1\n0.\nf 0 0\n1\n1 2 0 1\n0.25\n"));   // unknown subspace
+  EXPECT_THROW(parse("# symtype QS\n#!9\n1 2 1\n0 1\n2\n0.\n"));       // truncated energies
 }
 
 TEST(Store, fdm_ranges) {
