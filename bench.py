@@ -210,6 +210,11 @@ def bench_eig(args):
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
                     "bytes_per_launch": round(tot_bytes / nl), "launches_per_step": nl,
                     "avg_launch_us": round(avg_s * 1e6, 3)}
+        if dom == "k_col_prep":
+            roofline["note"] = ("k_col_prep moves few bytes per launch: its time is the chain of dependent memory "
+                                "round trips and block reductions of one reduced column (one workgroup per block), "
+                                "so its HBM fraction is low by construction; the bandwidth-bound kernel of the "
+                                "reduction is k_col_symv (roofline_other)")
     elif dom is not None:
         roofline = {"bound": "mfma", "kernel": dom, "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": None, "traffic": None}
