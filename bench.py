@@ -223,8 +223,10 @@ def bench_eig(args):
     for k, (tot_bytes, nl) in rb.items():
         if k != dom and k in kt and nl:
             tms, cnt = kt[k]
-            others[k] = {"achieved_GBs": round((tot_bytes / nl) / (tms / cnt * 1e-3) / 1e9, 2),
-                         "bytes_per_launch": round(tot_bytes / nl), "avg_launch_us": round(tms / cnt * 1e3, 3)}
+            gbs = (tot_bytes / nl) / (tms / cnt * 1e-3) / 1e9
+            others[k] = {"achieved_GBs": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic(k, args), "bytes_per_launch": round(tot_bytes / nl),
+                         "avg_launch_us": round(tms / cnt * 1e3, 3)}
 
     gflops = flops / (ms_step * 1e-3) / 1e9
     out = {
