@@ -168,8 +168,8 @@ def test_data_file_reader(tmp_path, name):
     dst = tmp_path / "data"
     r = run("--data", src, str(dst))
     assert r.returncode == 0, r.stdout + r.stderr
-    sym, channels, nmax, nsubs, nstates, perch, nops, ncoef, verdict = r.stdout.split()
-    assert verdict == "same"
+    sym, channels, nmax, nsubs, nstates, perch, nops, ncoef, outcome = r.stdout.split()
+    assert outcome == "same"
     with open(src) as f:
         text = f.read()
     assert sym == next(l.split()[-1] for l in text.splitlines() if "symtype" in l)
