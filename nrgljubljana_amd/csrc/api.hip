@@ -57,14 +57,29 @@ void PinnedBuf::release() {
   bytes = 0;
 }
 
-int upload_desc(nrg_ctx *ctx, const void *src, size_t bytes, hipStream_t st) {
+int stage_desc(nrg_ctx *ctx, size_t bytes, hipStream_t st, char **host) {
+  // both buffers are reused: make sure the previous upload and its readers have left them
+  NRG_HIP(hipStreamSynchronize(st));
   if (int r = ctx->desc.ensure(bytes)) return r;
   if (int r = ctx->hdesc.ensure(bytes)) return r;
-  // the staging buffer is reused: make sure the previous upload has left it
-  NRG_HIP(hipStreamSynchronize(st));
-  memcpy(ctx->hdesc.p, src, bytes);
+  *host = (char *)ctx->hdesc.p;
+  return NRG_OK;
+}
+
+int upload_staged(nrg_ctx *ctx, size_t bytes, hipStream_t st) {
+  if (bytes > ctx->hdesc.bytes || bytes > ctx->desc.bytes) {
+    set_error("upload_staged: more bytes than were staged");
+    return NRG_E_ARG;
+  }
   NRG_HIP(hipMemcpyAsync(ctx->desc.p, ctx->hdesc.p, bytes, hipMemcpyHostToDevice, st));
   return NRG_OK;
+}
+
+int upload_desc(nrg_ctx *ctx, const void *src, size_t bytes, hipStream_t st) {
+  char *h = nullptr;
+  if (int r = stage_desc(ctx, bytes, st, &h)) return r;
+  memcpy(h, src, bytes);
+  return upload_staged(ctx, bytes, st);
 }
 
 size_t KTimer::begin(hipStream_t st) {

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <memory>
 #include "../../include/nrgcore.h"
 
 namespace nrg {
@@ -126,6 +127,8 @@ struct nrg_ctx {
   static constexpr int NAUX = 3;
   hipStream_t aux[NAUX] = {};
   hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {};
+  // the eigensolver's host-side work lists, kept between calls so that their memory is reused
+  std::shared_ptr<void> syev_cache;
 };
 
 namespace nrg {
@@ -151,4 +154,9 @@ int broaden_run(nrg_ctx *ctx, const double *pos_e, const double *pos_w, int64_t 
                 double *out_pos, double *out_neg, hipStream_t st);
 // Uploads `bytes` of host data through the pinned staging buffer into ctx->desc at byte offset 0.
 int upload_desc(nrg_ctx *ctx, const void *src, size_t bytes, hipStream_t st);
+// The same in two steps, for tables written straight into the staging buffer: stage_desc returns
+// the buffer (`bytes` long, once the previous upload has left it), upload_staged copies its first
+// `bytes` into ctx->desc.
+int stage_desc(nrg_ctx *ctx, size_t bytes, hipStream_t st, char **host);
+int upload_staged(nrg_ctx *ctx, size_t bytes, hipStream_t st);
 }  // namespace nrg

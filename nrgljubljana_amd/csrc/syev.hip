@@ -1600,6 +1600,32 @@ struct SyevPlan {
   std::vector<size_t> s2_items, s2_cnt;   // trailing rank-2k updates per (panel slot, group): [q * ngrp + g]
 };
 
+// The work lists and product tables of a call, kept in the context for the next call: the lists of
+// a 256-block batch take some 4 MB, and fresh memory for them each call costs more host time (page
+// faults) than building them.
+struct SyevCache {
+  SyevPlan pl;
+  std::vector<GemmDesc> g64, gref;
+  std::vector<GemmDesc32> g32;
+};
+
+// An empty plan that keeps the memory of the previous call's lists.
+static SyevCache &syev_cache(nrg_ctx *ctx) {
+  if (!ctx->syev_cache) ctx->syev_cache = std::make_shared<SyevCache>();
+  SyevCache &c = *static_cast<SyevCache *>(ctx->syev_cache.get());
+  std::vector<int> ibuf = std::move(c.pl.ibuf);
+  std::vector<GemmDesc> gd = std::move(c.pl.gd);
+  c.pl = SyevPlan();
+  ibuf.clear();
+  gd.clear();
+  c.pl.ibuf = std::move(ibuf);
+  c.pl.gd = std::move(gd);
+  c.g64.clear();
+  c.gref.clear();
+  c.g32.clear();
+  return c;
+}
+
 // Panels reduced by the column kernels: those that start with more than TAIL trailing rows. The
 // trailing matrix of order <= TAIL that remains is reduced by k_tail.
 static int main_panels(int64_t n) {
@@ -1653,7 +1679,11 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   // bandwidth-bound symmetric product. Launches are listed slot by slot, column by column, group by
   // group, the order in which they are issued.
   pl.grp.assign(nblk, 0);
-  for (size_t k = 0; k < order.size(); k++) pl.grp[order[k]] = (int)(k % pl.ngrp);
+  std::vector<std::vector<int>> members(pl.ngrp);   // the blocks of each group, largest first
+  for (size_t k = 0; k < order.size(); k++) {
+    pl.grp[order[k]] = (int)(k % pl.ngrp);
+    members[k % pl.ngrp].push_back(order[k]);
+  }
   pl.cols.clear();
   for (int q = 0; q < pl.Pmax; q++) {
     auto panel = [&](int b) { return panel_in_slot(q, pl.bd[b].P); };
@@ -1665,20 +1695,24 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     for (int b : order) if (panel(b) >= 0) wmax = std::max(wmax, width(b));
     for (int jl = 0; jl <= wmax; jl++)
       for (int g = 0; g < pl.ngrp; g++) {
+        // A block works on column jl in slot q while it has a panel there (P_b > q) that is wider
+        // than jl. Both bounds grow with the block's order, so the blocks that do are a leading
+        // run of the group's list, largest first; the list ends at the first one that does not.
+        const std::vector<int> &mb = members[g];
         SyevPlan::ColLaunch c;
         c.p = q; c.jl = jl; c.g = g;
         c.prep_off = I.size();
         c.rows = 0;
-        for (int b : order)
-          if (pl.grp[b] == g && panel(b) >= 0 && jl <= width(b)) {
-            I.push_back(col_entry(b, panel(b)));
-            c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * panel(b) - jl));
-          }
+        size_t np = 0;
+        for (; np < mb.size() && panel(mb[np]) >= 0 && jl <= width(mb[np]); np++) {
+          const int b = mb[np];
+          I.push_back(col_entry(b, panel(b)));
+          c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * panel(b) - jl));
+        }
         c.prep_cnt = I.size() - c.prep_off;
         if (!c.prep_cnt) continue;   // every symv block is a prep block too
         c.symv_off = I.size();
-        for (int b : order)
-          if (pl.grp[b] == g && panel(b) >= 0 && jl < width(b)) I.push_back(col_entry(b, panel(b)));
+        for (size_t k = 0; k < np && jl < width(mb[k]); k++) I.push_back(col_entry(mb[k], panel(mb[k])));
         c.nact = I.size() - c.symv_off;
         c.tpre_off = I.size();
         int64_t tiles = 0;
@@ -1719,24 +1753,34 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   pl.ch_off.assign(pl.Lmax, 0); pl.ch_cnt.assign(pl.Lmax, 0);
   pl.rc_off.assign(pl.Lmax, 0); pl.rc_cnt.assign(pl.Lmax, 0);
   pl.fc_off.assign(pl.Lmax, 0); pl.fc_cnt.assign(pl.Lmax, 0);
+  // chunks of every merge: 256-wide, RCH-wide (roots) and FILLC-wide (columns of S), each list
+  // (merge, chunk) pairs merge by merge; sized first and written in place
+  std::vector<int> sizes_here;
   for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
-    std::vector<int> sizes_here;
+    sizes_here.resize(pl.mrg_cnt[depth]);
+    size_t nch = 0, nrc = 0, nfc = 0;
     for (size_t mi = 0; mi < pl.mrg_cnt[depth]; mi++) {
       const int b = I[pl.mrg_off[depth] + 2 * mi], q = I[pl.mrg_off[depth] + 2 * mi + 1];
-      sizes_here.push_back((int)((((int64_t)(q + 1) * n[b]) >> depth) - (((int64_t)q * n[b]) >> depth)));
+      const int m = (int)((((int64_t)(q + 1) * n[b]) >> depth) - (((int64_t)q * n[b]) >> depth));
+      sizes_here[mi] = m;
+      nch += (m + 255) / 256;
+      nrc += (m + RCH - 1) / RCH;
+      nfc += (m + FILLC - 1) / FILLC;
     }
     pl.ch_off[depth] = I.size();
-    for (size_t mi = 0; mi < sizes_here.size(); mi++)
-      for (int c = 0; c < (sizes_here[mi] + 255) / 256; c++) push2((int)mi, c);
-    pl.ch_cnt[depth] = (I.size() - pl.ch_off[depth]) / 2;
-    pl.rc_off[depth] = I.size();
-    for (size_t mi = 0; mi < sizes_here.size(); mi++)
-      for (int c = 0; c < (sizes_here[mi] + RCH - 1) / RCH; c++) push2((int)mi, c);
-    pl.rc_cnt[depth] = (I.size() - pl.rc_off[depth]) / 2;
-    pl.fc_off[depth] = I.size();
-    for (size_t mi = 0; mi < sizes_here.size(); mi++)
-      for (int c = 0; c < (sizes_here[mi] + FILLC - 1) / FILLC; c++) push2((int)mi, c);
-    pl.fc_cnt[depth] = (I.size() - pl.fc_off[depth]) / 2;
+    pl.ch_cnt[depth] = nch;
+    pl.rc_off[depth] = pl.ch_off[depth] + 2 * nch;
+    pl.rc_cnt[depth] = nrc;
+    pl.fc_off[depth] = pl.rc_off[depth] + 2 * nrc;
+    pl.fc_cnt[depth] = nfc;
+    I.resize(pl.fc_off[depth] + 2 * nfc);
+    int *wch = I.data() + pl.ch_off[depth], *wrc = I.data() + pl.rc_off[depth], *wfc = I.data() + pl.fc_off[depth];
+    for (size_t mi = 0; mi < sizes_here.size(); mi++) {
+      const int m = sizes_here[mi];
+      for (int c = 0; c < (m + 255) / 256; c++) { *wch++ = (int)mi; *wch++ = c; }
+      for (int c = 0; c < (m + RCH - 1) / RCH; c++) { *wrc++ = (int)mi; *wrc++ = c; }
+      for (int c = 0; c < (m + FILLC - 1) / FILLC; c++) { *wfc++ = (int)mi; *wfc++ = c; }
+    }
   }
   // FP64 back-transformation: super-panel offsets, Y columns and T factors
   pl.toff.assign(nblk, 0);
@@ -2064,7 +2108,8 @@ static int finish_flags(nrg_ctx *ctx, int nblk, int *info, hipStream_t st) {
 
 int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                   double *w, int *info, hipStream_t st) {
-  SyevPlan pl;
+  SyevCache &cache = syev_cache(ctx);
+  SyevPlan &pl = cache.pl;
   build_plan(pl, nblk, n, off_H, off_w, true, 8, red_groups());
   Carver cv;
   const size_t oZ0 = cv.take<double>(pl.sumn2), oZ1 = cv.take<double>(pl.sumn2), oS = cv.take<double>(pl.sumn2);
@@ -2077,7 +2122,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   if (int r = ctx->ws.ensure(cv.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
   build_dc_gemms(pl, Z0, Z1, S);
-  std::vector<GemmDesc> gdT;
+  std::vector<GemmDesc> &gdT = cache.g64;
   build_red_bt_gemms<double>(pl, gdT, H, ctx->ws.as<double>(oWp), S, Z1, Z0, ctx->ws.as<double>(oGb),
                              ctx->ws.as<double>(oTb), ctx->ws.as<double>(oW2));
   // descriptor upload: [BlkDesc][GemmDesc D&C][GemmDesc work][toff][ints]
@@ -2086,16 +2131,17 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   const size_t oBD = 0, oP = (bBD + 255) & ~(size_t)255, oQ = oP + ((bP + 255) & ~(size_t)255);
   const size_t oT = oQ + ((bQ + 255) & ~(size_t)255);
   const size_t oI = oT + ((bT + 255) & ~(size_t)255);
-  std::vector<char> blob(oI + bI);
-  memcpy(blob.data() + oBD, pl.bd.data(), bBD);
-  if (bP) memcpy(blob.data() + oP, pl.gd.data(), bP);
-  if (bQ) memcpy(blob.data() + oQ, gdT.data(), bQ);
-  if (bT) memcpy(blob.data() + oT, pl.toff.data(), bT);
-  memcpy(blob.data() + oI, pl.ibuf.data(), bI);
   if (int r = ctx->flags.ensure(sizeof(int) * nblk)) return r;
   if (int r = ctx->hflags.ensure(sizeof(int) * nblk)) return r;
   NRG_HIP(hipEventRecord(ctx->ev[0], st));
-  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
+  char *blob = nullptr;
+  if (int r = stage_desc(ctx, oI + bI, st, &blob)) return r;
+  memcpy(blob + oBD, pl.bd.data(), bBD);
+  if (bP) memcpy(blob + oP, pl.gd.data(), bP);
+  if (bQ) memcpy(blob + oQ, gdT.data(), bQ);
+  if (bT) memcpy(blob + oT, pl.toff.data(), bT);
+  memcpy(blob + oI, pl.ibuf.data(), bI);
+  if (int r = upload_staged(ctx, oI + bI, st)) return r;
   NRG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * nblk, st));
   NRG_HIP(hipMemsetAsync(ctx->ws.as<double>(oamax), 0, sizeof(double) * nblk, st));
   SyevArgs<double> a;
@@ -2153,7 +2199,8 @@ static int refine_sweeps() {
 
 int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                    double *w, int *info, hipStream_t st) {
-  SyevPlan pl;
+  SyevCache &cache = syev_cache(ctx);
+  SyevPlan &pl = cache.pl;
   build_plan(pl, nblk, n, off_H, off_w, false, 4, red_groups());
   // per-block chunk items for the elementwise refinement kernels
   const size_t rc_off = pl.ibuf.size();
@@ -2182,14 +2229,14 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   if (int r = ctx->ws2.ensure(cf.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
   build_dc_gemms(pl, Z0, Z1, S);
-  std::vector<GemmDesc32> gdT;
+  std::vector<GemmDesc32> &gdT = cache.g32;
   float *A32 = ctx->ws2.as<float>(fA);
   build_red_bt_gemms<float>(pl, gdT, A32, ctx->ws2.as<float>(fWp), ctx->ws2.as<float>(fYm), ctx->ws2.as<float>(fYh),
                             ctx->ws2.as<float>(fZt), ctx->ws2.as<float>(fGb), ctx->ws2.as<float>(fTb),
                             ctx->ws2.as<float>(fW2));
   // refinement products: AX = A X; S = X^T AX and R = I - X^T X; Xn = X (I + E). Both sweep
   // directions (X in Z0 -> Xn in Z1, then back) are prepared.
-  std::vector<GemmDesc> gref;
+  std::vector<GemmDesc> &gref = cache.gref;
   double *Xbuf[2] = {Z0, Z1};
   double *AX = ctx->ws.as<double>(oAX), *Rm = ctx->ws.as<double>(oR), *Em = ctx->ws.as<double>(oE);
   size_t rg_off[2][4], rg_cnt[2][4];
@@ -2231,17 +2278,18 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   const size_t bT = sizeof(int64_t) * pl.toff.size(), bI = sizeof(int) * pl.ibuf.size();
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t oBD = 0, oP = al(bBD), oQ = oP + al(bP), oRr = oQ + al(bQ), oT = oRr + al(bR), oI = oT + al(bT);
-  std::vector<char> blob(oI + bI);
-  memcpy(blob.data() + oBD, pl.bd.data(), bBD);
-  if (bP) memcpy(blob.data() + oP, pl.gd.data(), bP);
-  if (bQ) memcpy(blob.data() + oQ, gdT.data(), bQ);
-  if (bR) memcpy(blob.data() + oRr, gref.data(), bR);
-  if (bT) memcpy(blob.data() + oT, pl.toff.data(), bT);
-  memcpy(blob.data() + oI, pl.ibuf.data(), bI);
   if (int r = ctx->flags.ensure(sizeof(int) * nblk)) return r;
   if (int r = ctx->hflags.ensure(sizeof(int) * nblk)) return r;
   NRG_HIP(hipEventRecord(ctx->ev[0], st));
-  if (int r = upload_desc(ctx, blob.data(), blob.size(), st)) return r;
+  char *blob = nullptr;
+  if (int r = stage_desc(ctx, oI + bI, st, &blob)) return r;
+  memcpy(blob + oBD, pl.bd.data(), bBD);
+  if (bP) memcpy(blob + oP, pl.gd.data(), bP);
+  if (bQ) memcpy(blob + oQ, gdT.data(), bQ);
+  if (bR) memcpy(blob + oRr, gref.data(), bR);
+  if (bT) memcpy(blob + oT, pl.toff.data(), bT);
+  memcpy(blob + oI, pl.ibuf.data(), bI);
+  if (int r = upload_staged(ctx, oI + bI, st)) return r;
   NRG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * nblk, st));
   NRG_HIP(hipMemsetAsync(ctx->ws.as<double>(oamax), 0, sizeof(double) * nblk, st));
   const BlkDesc *dbd = (const BlkDesc *)(ctx->desc.as<char>(oBD));
