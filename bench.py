@@ -621,6 +621,73 @@ def bench_ft(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_temp(args):
+    """chi(T) temperature sum of one synthetic step (nrg_temp_sum, Algo_CHIT over kept x kept);
+    value = elements/s."""
+    import torch
+    import oracle
+    from nrgljubljana_amd import Context, FdmPair, workloads
+    from nrgljubljana_amd.core import NRG_TEMP_CHIT
+    pairs = workloads.dmnrg_step(npairs=args.npairs)
+    nel = sum(p["K_i"] * p["K_j"] for p in pairs)
+    dev = torch.device("cuda:0")
+    T, Z = 0.5, 40.0
+    keep, structs = [], []
+    for p in pairs:
+        t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev) for k in ("E_i", "E_j", "op")}
+        keep.append(t)
+        structs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], t["E_i"].data_ptr(), t["E_j"].data_ptr(),
+                               t["op"].data_ptr(), t["op"].data_ptr(), p["dim_i"], None, 0, None, 0, 1.0))
+    out = torch.zeros(1, dtype=torch.float64, device=dev)
+    ctx = Context(0)
+
+    def step():
+        out.zero_()
+        ctx.temp_sum(NRG_TEMP_CHIT, structs, T, Z, out)
+    ms = timed_calls(step, args.steps, args.warmup, torch.cuda.synchronize)
+    ctx.kernel_timing(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    log(f"temp step: {len(pairs)} pairs, {nel / 1e6:.2f} M elements, {ms:.3f} ms per call; "
+        + ", ".join(f"{k} {v[0] / 2:.3f} ms" for k, v in kt.items()))
+    tms, cnt = kt["k_temp_sum"]
+    abytes = 16.0 * nel + 8.0 * sum(p["K_i"] + p["K_j"] for p in pairs)
+    ach = abytes / (tms / cnt * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_temp_sum", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "note": "algorithmic bytes = 16 B per element (op1, op2) + 8 B per energy"}
+    cpu = None
+    if not args.no_cpu:
+        hs, hold, el = [], [], 0
+        for p in sorted(pairs, key=lambda q: q["K_i"] * q["K_j"])[:len(pairs) // 2]:
+            arrs = {k: np.ascontiguousarray(p[k]) for k in ("E_i", "E_j", "op")}
+            hold.append(arrs)
+            hs.append(FdmPair(p["dim_i"], p["dim_j"], p["K_i"], p["K_j"], arrs["E_i"].ctypes.data,
+                              arrs["E_j"].ctypes.data, arrs["op"].ctypes.data, arrs["op"].ctypes.data, p["dim_i"],
+                              None, 0, None, 0, 1.0))
+            el += p["K_i"] * p["K_j"]
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and reps < 5000:
+            oracle.temp_sum("chit", hs, T, Z)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * el / dt / 1e6, 3), "unit": "Melements/s", "cores": 1, "kind": "port",
+               "sample": f"the {len(hs)} smallest pairs ({el / 1e6:.2f} M elements) {reps} times, the reference's "
+                         f"loop, {dt:.1f} s"}
+    res = {"metric": "chi(T) temperature sum, elements per second", "value": round(nel / ms / 1e3, 2),
+           "unit": "Melements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"one chi(T) step, {len(pairs)} subspace pairs, kept dims log-uniform [16,1000], "
+                                  "seed 20261208", "elements": nel},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def bench_mats(args):
     """Matsubara Green's function of one synthetic FDM step (nrg_fdm_mats, 100 frequencies, the
     reference's default P.mats); value = (element, frequency) evaluations per second."""
@@ -1170,8 +1237,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "cfs", "ft", "mats", "ham", "step"],
-                    default="eig")
+    ap.add_argument("--workload", default="eig",
+                    choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "cfs", "ft", "mats", "temp", "ham", "step"])
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=16, help="eig: smallest block order")
@@ -1186,7 +1253,8 @@ def main():
               "design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
-     "dm": bench_dm, "dmnrg": bench_dmnrg, "cfs": bench_cfs, "ft": bench_ft, "mats": bench_mats, "ham": bench_ham, "step": bench_step}[args.workload](args)
+     "dm": bench_dm, "dmnrg": bench_dmnrg, "cfs": bench_cfs, "ft": bench_ft, "mats": bench_mats,
+     "temp": bench_temp, "ham": bench_ham, "step": bench_step}[args.workload](args)
 
 
 if __name__ == "__main__":
