@@ -42,8 +42,10 @@ extern "C" {
 
 typedef struct nrg_ctx nrg_ctx;
 
-/* Opaque context: owns the device workspace, descriptor staging and timers for
- * one device. A context is not thread-safe; use one per host thread. */
+/* Opaque context: owns the device workspace, descriptor staging, the eigensolver's
+ * host-side work lists and timers for one device; all of them grow to the largest
+ * call seen and are kept until nrg_ctx_destroy. A context is not thread-safe; use
+ * one per host thread. */
 int nrg_ctx_create(nrg_ctx **ctx, int device);
 int nrg_ctx_destroy(nrg_ctx *ctx);
 const char *nrg_last_error(void);
