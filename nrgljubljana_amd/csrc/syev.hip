@@ -1615,11 +1615,14 @@ static SyevCache &syev_cache(nrg_ctx *ctx) {
   SyevCache &c = *static_cast<SyevCache *>(ctx->syev_cache.get());
   std::vector<int> ibuf = std::move(c.pl.ibuf);
   std::vector<GemmDesc> gd = std::move(c.pl.gd);
+  std::vector<SyevPlan::ColLaunch> cols = std::move(c.pl.cols);
   c.pl = SyevPlan();
   ibuf.clear();
   gd.clear();
+  cols.clear();
   c.pl.ibuf = std::move(ibuf);
   c.pl.gd = std::move(gd);
+  c.pl.cols = std::move(cols);
   c.g64.clear();
   c.gref.clear();
   c.g32.clear();
@@ -1685,6 +1688,19 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     members[k % pl.ngrp].push_back(order[k]);
   }
   pl.cols.clear();
+  // Consecutive launches of a group within a slot mostly have the same block list (every active
+  // block works on every column of its panel), and often the same tile offsets: a list equal to
+  // the group's previous list of that kind is not stored again.
+  std::vector<int> tmp;
+  struct Last { size_t off = 0, cnt = 0; };
+  std::vector<Last> last_blk(pl.ngrp), last_tile(pl.ngrp);
+  auto share = [&](Last &last) -> size_t {
+    if (last.cnt == tmp.size() && std::equal(tmp.begin(), tmp.end(), I.begin() + last.off)) return last.off;
+    last.off = I.size();
+    last.cnt = tmp.size();
+    I.insert(I.end(), tmp.begin(), tmp.end());
+    return last.off;
+  };
   for (int q = 0; q < pl.Pmax; q++) {
     auto panel = [&](int b) { return panel_in_slot(q, pl.bd[b].P); };
     auto width = [&](int b) {
@@ -1701,28 +1717,38 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
         const std::vector<int> &mb = members[g];
         SyevPlan::ColLaunch c;
         c.p = q; c.jl = jl; c.g = g;
-        c.prep_off = I.size();
         c.rows = 0;
         size_t np = 0;
         for (; np < mb.size() && panel(mb[np]) >= 0 && jl <= width(mb[np]); np++) {
           const int b = mb[np];
-          I.push_back(col_entry(b, panel(b)));
           c.rows = std::max(c.rows, (int)(n[b] - (int64_t)NB * panel(b) - jl));
         }
-        c.prep_cnt = I.size() - c.prep_off;
-        if (!c.prep_cnt) continue;   // every symv block is a prep block too
-        c.symv_off = I.size();
-        for (size_t k = 0; k < np && jl < width(mb[k]); k++) I.push_back(col_entry(mb[k], panel(mb[k])));
-        c.nact = I.size() - c.symv_off;
-        c.tpre_off = I.size();
+        if (!np) continue;   // every symv block is a prep block too
+        size_t ns = 0;
+        while (ns < np && jl < width(mb[ns])) ns++;
+        // block lists: the symv blocks are the leading ns of the np prep blocks
+        tmp.clear();
+        for (size_t k = 0; k < np; k++) tmp.push_back(col_entry(mb[k], panel(mb[k])));
+        c.prep_off = share(last_blk[g]);
+        c.prep_cnt = np;
+        if (ns == np) {
+          c.symv_off = c.prep_off;
+        } else {
+          tmp.resize(ns);
+          c.symv_off = share(last_blk[g]);
+        }
+        c.nact = ns;
+        // tile offsets of the symv blocks
+        tmp.clear();
         int64_t tiles = 0;
-        for (size_t k = 0; k < c.nact; k++) {
-          const int b = I[c.symv_off + k] & COL_BLK_MASK;
+        for (size_t k = 0; k < ns; k++) {
+          const int b = mb[k];
           const int64_t t = n[b] - (int64_t)NB * panel(b) - jl - 1;
           const int64_t T = (t + 63) / 64;
-          I.push_back((int)tiles);
+          tmp.push_back((int)tiles);
           tiles += T * (T + 1) / 2 + (jl > 0 ? T : 0);   // + one dot-product workgroup per segment
         }
+        c.tpre_off = share(last_tile[g]);
         c.tiles = (size_t)tiles;
         pl.cols.push_back(c);
       }
