@@ -10,7 +10,7 @@ resident in HBM before the timed region starts.
 value = algorithmic GFLOP/s = sum_b (10/3) n_b^3 / time per step (the FLOP count of Householder
 reduction + back-transformation, fixed for every algorithm and precision).
 
-Prints one JSON line. Rank 0 only; this build runs on one GPU.
+Prints one JSON line. One process on one GPU.
 """
 import argparse
 import json
@@ -1249,8 +1249,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the eigensolver shards by blocks across GPUs in "
-              "design only); run with --gpus 1", file=sys.stderr)
+        print("bench.py: runs on one GPU only (sharding the blocks of a step across GPUs is a design, not "
+              "built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
      "dm": bench_dm, "dmnrg": bench_dmnrg, "cfs": bench_cfs, "ft": bench_ft, "mats": bench_mats,

@@ -3,7 +3,7 @@
 // Store / Sub (c++/store.hpp:19-96), grand_canonical_Z, calc_ZnD and fdm_thermodynamics
 // (c++/measurements.hpp:51-163). The reference accumulates Z_n^D, Z_{ZZG} and the energy moments in
 // 400-bit GMP floats and reads them back with mpf_get_d (which truncates toward zero). GMP is not
-// part of this build: the sums here are kept in IEEE binary128 (__float128, 113-bit significand). The
+// used here: the sums are kept in IEEE binary128 (__float128, 113-bit significand). The
 // summands are the same doubles (mult * exp(...) formed in double, as the reference forms them before
 // mpf_set_d), every addend is positive in the Z sums, and the read-back truncates toward zero as
 // mpf_get_d does, so the double results agree with the 400-bit ones unless the exact sum lies within
