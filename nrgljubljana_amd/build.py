@@ -34,7 +34,27 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
     os.replace(OUT + ".tmp", OUT)
+    build_run(verbose)
     return OUT
+
+
+RUN_OUT = os.path.join(HERE, "libnrgrun.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_run(verbose=False):
+    """libnrgrun.so: the whole-calculation driver (host/nrg_run.cpp, C++ on the host) over libnrgcore.so."""
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-unused-function", "-D__HIP_PLATFORM_AMD__",
+           "-I" + os.path.join(ROCM, "include"), os.path.join(HERE, "host", "nrg_run.cpp"), "-o", RUN_OUT + ".tmp",
+           "-L" + HERE, "-lnrgcore", "-Wl,-rpath,$ORIGIN", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-lquadmath", "-ldl"]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("building libnrgrun.so failed:\n" + r.stdout + r.stderr)
+    os.replace(RUN_OUT + ".tmp", RUN_OUT)
+    return RUN_OUT
 
 
 if __name__ == "__main__":

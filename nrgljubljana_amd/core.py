@@ -295,3 +295,36 @@ def _ptr(t):
 
 def phase_name(phase):
     return lib().nrg_phase_name(int(phase)).decode()
+
+
+# ---- a whole NRG calculation (libnrgrun.so, C ABI declared in include/nrgrun.h)
+RUN_LIB_PATH = os.path.join(_HERE, "libnrgrun.so")
+TABLES_DIR = os.path.join(_HERE, "tables")
+RUN_EXPORTED = ["nrg_run_dir", "nrg_run_error"]
+_runlib = None
+
+
+def runlib():
+    """Load libnrgrun.so (raises if it has not been built)."""
+    global _runlib
+    if _runlib is not None:
+        return _runlib
+    if not os.path.exists(RUN_LIB_PATH):
+        raise RuntimeError(f"{RUN_LIB_PATH} is missing: build it with nrgljubljana_amd.build.build()")
+    L = ctypes.CDLL(RUN_LIB_PATH)
+    L.nrg_run_dir.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p]
+    L.nrg_run_dir.restype = ctypes.c_int
+    L.nrg_run_error.restype = ctypes.c_char_p
+    _runlib = L
+    return L
+
+
+def run_nrg(directory, device=0, log_path=None, tables_dir=None):
+    """Run the NRG calculation whose `param` and `data` files are in `directory` (as the reference's
+    run_nrg_master(dir), c++/nrg-lib.cc:22-32); the output files are written there. Raises NrgError
+    with the library's message on failure."""
+    L = runlib()
+    r = L.nrg_run_dir(os.fspath(directory).encode(), (tables_dir or TABLES_DIR).encode(), int(device),
+                      None if log_path is None else os.fspath(log_path).encode())
+    if r != 0:
+        raise NrgError(f"nrg_run_dir failed: {L.nrg_run_error().decode()}")

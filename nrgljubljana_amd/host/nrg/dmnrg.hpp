@@ -77,14 +77,17 @@ inline DensMatElements calc_densitymatrix_iterN(const DiagInfo &diag, const Dens
   return rhoPrev;
 }
 
-// c++/dmnrg.hpp:118-133; diags[N] holds the eigenvectors of step N
+// c++/dmnrg.hpp:118-133; diags[N] holds the eigenvectors of step N; `keep` (if given) receives
+// rho(N-1) for every step, as the reference saves it (rhoPrev.save(N-1, P, filename))
 inline void calc_densitymatrix(DensMatElements &rho, const Store &store, const Symmetry &sym,
-                               const std::vector<const DiagInfo *> &diags, size_t Ninit, size_t Nmax, Device &dev) {
+                               const std::vector<const DiagInfo *> &diags, size_t Ninit, size_t Nmax, Device &dev,
+                               const std::function<void(size_t, const DensMatElements &)> &keep = nullptr) {
   check_trace_rho(rho, sym.multfnc());
   Gemm3Batch batch(dev);
   for (size_t N = Nmax - 1; N > Ninit; N--) {
     auto rhoPrev = calc_densitymatrix_iterN(*diags[N], rho, N, store, sym, batch);
     check_trace_rho(rhoPrev, sym.multfnc());
+    if (keep) keep(N - 1, rhoPrev);
     rho.swap(rhoPrev);
   }
 }

@@ -77,6 +77,7 @@ class StepStore {
 
   // DensMatElements::save(N, P, prefix): the store takes the matrices over
   void save_rho(size_t N, const std::string &prefix, DensMatElements &&rho) {
+    check_prefix(prefix);
     const Key k{N, prefix};
     drop_rho(k);
     RhoEntry &e = rhos_[k];
@@ -89,6 +90,7 @@ class StepStore {
   }
   // DensMatElements::load(N, P, prefix, remove_files): a device copy, or the store's matrices on remove
   DensMatElements load_rho(size_t N, const std::string &prefix, bool remove = false) {
+    check_prefix(prefix);
     const Key k{N, prefix};
     auto it = rhos_.find(k);
     if (it == rhos_.end())
@@ -122,6 +124,10 @@ class StepStore {
   size_t budget() const { return budget_; }
 
  private:
+  // the empty prefix names the eigenvectors of a step in the shared key space
+  static void check_prefix(const std::string &prefix) {
+    if (prefix.empty()) throw std::invalid_argument("StepStore: density matrices need a non-empty prefix");
+  }
   struct Key {
     size_t N;
     std::string prefix;   // "" for eigenvectors
