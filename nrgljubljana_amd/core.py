@@ -327,4 +327,4 @@ def run_nrg(directory, device=0, log_path=None, tables_dir=None):
     r = L.nrg_run_dir(os.fspath(directory).encode(), (tables_dir or TABLES_DIR).encode(), int(device),
                       None if log_path is None else os.fspath(log_path).encode())
     if r != 0:
-        raise NrgError(f"nrg_run_dir failed: {L.nrg_run_error().decode()}")
+        raise RuntimeError(f"nrg_run_dir failed: {L.nrg_run_error().decode()}")
