@@ -249,6 +249,17 @@ int nrg_temp_sum(nrg_ctx *ctx, int kind, const nrg_fdm_pair *pairs, int npairs, 
                  double *out, void *stream);
 
 /*
+ * nrg_temp_sum without any device allocation in the call. It has no counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight serving loops). Same
+ * arguments, semantics and results as nrg_temp_sum; the descriptors and partial sums live in a
+ * static 4 MiB array per GPU (its address is looked up once, in the first nrg_ctx_create on each
+ * device), so a step whose descriptors need more is rejected with NRG_E_ARG. Every call on a GPU
+ * shares that array: calls that overlap on one GPU race. The call synchronises `stream`.
+ */
+int nrg_temp_sum_noalloc(nrg_ctx *ctx, int kind, const nrg_fdm_pair *pairs, int npairs, double temperature,
+                         double Z, double *out, void *stream);
+
+/*
  * DM-NRG spectral weights of all subspace pairs of one step, binned: replaces the pair loop of
  * BaseSpectrum::calc (c++/algo.hpp:55-67) driving Algo_DMNRG::calc (c++/algo_DMNRG.hpp:30-52)
  * and ChainBinning::add (c++/spectrum.hpp:19-28). Same pair descriptors as nrg_fdm_bin, except that

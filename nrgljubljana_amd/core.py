@@ -19,7 +19,7 @@ NRG_GEMM_OZAKI = 1
 
 EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
             "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_cfs_bin", "nrg_ft_bin", "nrg_fdm_mats", "nrg_dmnrg_mats",
-            "nrg_ft_mats", "nrg_temp_sum", "nrg_broaden", "nrg_hamiltonian",
+            "nrg_ft_mats", "nrg_temp_sum", "nrg_temp_sum_noalloc", "nrg_broaden", "nrg_hamiltonian",
             "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
             "nrg_kernel_time", "nrg_kernel_names"]
 
@@ -109,6 +109,8 @@ def lib():
     L.nrg_temp_sum.argtypes = [vp, ctypes.c_int, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.c_double,
                                ctypes.c_double, vp, vp]
     L.nrg_temp_sum.restype = ctypes.c_int
+    L.nrg_temp_sum_noalloc.argtypes = L.nrg_temp_sum.argtypes
+    L.nrg_temp_sum_noalloc.restype = ctypes.c_int
     L.nrg_broaden.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_double,
                               ctypes.c_double, vp, vp, vp]
     L.nrg_broaden.restype = ctypes.c_int

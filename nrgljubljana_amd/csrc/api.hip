@@ -164,6 +164,7 @@ int nrg_ctx_create(nrg_ctx **out, int device) {
       return NRG_E_NODEV;
     }
     NRG_HIP(hipSetDevice(device));
+    if (int r = scratch_setup(device)) return r;
     nrg_ctx *c = new nrg_ctx();
     c->device = device;
     for (int i = 0; i <= NPHASE; i++) {

@@ -132,6 +132,8 @@ struct nrg_ctx {
 };
 
 namespace nrg {
+// looks up the device's address of the no-allocation scratch array (noalloc.hip), once per device
+int scratch_setup(int device);
 // creates ctx->aux / fork_ev / join_ev once
 int ensure_aux_streams(nrg_ctx *ctx);
 int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,

@@ -100,6 +100,15 @@ void replicas_reduce(nrg_ctx *ctx, hipStream_t st, const double *priv, int64_t n
                                                         0, st, priv, n, nh, o0, o1, o2, o3));
 }
 
+void replicas_zero_untimed(hipStream_t st, double *p, size_t n) {
+  hipLaunchKernelGGL(k_fdm_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st, p, n);
+}
+
+void replicas_reduce_untimed(hipStream_t st, const double *priv, int64_t n, int nh, int nout, double *o0, double *o1,
+                             double *o2, double *o3) {
+  hipLaunchKernelGGL(k_fdm_reduce, dim3((unsigned)((n + 255) / 256), nout), dim3(256), 0, st, priv, n, nh, o0, o1, o2, o3);
+}
+
 int check_pairs(const char *who, const nrg_fdm_pair *pairs, int npairs, RhoUse use) {
   for (int p = 0; p < npairs; p++) {
     const nrg_fdm_pair &q = pairs[p];

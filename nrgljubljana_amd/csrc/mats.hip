@@ -345,6 +345,17 @@ __global__ __launch_bounds__(256) void k_temp_sum(TempArgs a, const int2 *items)
   if (threadIdx.x == 0) atomicAdd(a.priv + blockIdx.x % FDM_COPIES, red[0] + red[1] + red[2] + red[3]);
 }
 
+// k_temp_sum over `nitems` (pair, tile) items, partial sums into priv[0, FDM_COPIES), no timing
+void temp_launch(hipStream_t st, int kind, const FdmPairDev *pdev, const int2 *idev, unsigned nitems, double beta,
+                 double Z, double *priv) {
+  if (!nitems) return;
+  const TempArgs ta{pdev, beta, Z, priv};
+  if (kind == TEMP_GT) hipLaunchKernelGGL(k_temp_sum<TEMP_GT>, dim3(nitems), dim3(256), 0, st, ta, idev);
+  else if (kind == TEMP_I1T) hipLaunchKernelGGL(k_temp_sum<TEMP_I1T>, dim3(nitems), dim3(256), 0, st, ta, idev);
+  else if (kind == TEMP_I2T) hipLaunchKernelGGL(k_temp_sum<TEMP_I2T>, dim3(nitems), dim3(256), 0, st, ta, idev);
+  else hipLaunchKernelGGL(k_temp_sum<TEMP_CHIT>, dim3(nitems), dim3(256), 0, st, ta, idev);
+}
+
 int temp_run(nrg_ctx *ctx, int kind, const nrg_fdm_pair *pairs, int npairs, double temperature, double Z, double *out,
              hipStream_t st) {
   if (kind < TEMP_GT || kind > TEMP_CHIT || npairs < 0 || (npairs > 0 && !pairs) || !out || !(temperature > 0) ||

@@ -50,6 +50,14 @@ void replicas_zero(nrg_ctx *ctx, hipStream_t st, double *p, size_t n);
 void replicas_reduce(nrg_ctx *ctx, hipStream_t st, const double *priv, int64_t n, int nh, int nout, double *o0,
                      double *o1, double *o2, double *o3);
 
+// the same two launches without the per-kernel timers (they create events on first use)
+void replicas_zero_untimed(hipStream_t st, double *p, size_t n);
+void replicas_reduce_untimed(hipStream_t st, const double *priv, int64_t n, int nh, int nout, double *o0, double *o1,
+                             double *o2, double *o3);
+// k_temp_sum over nitems (pair, tile) items into FDM_COPIES partial sums at priv (mats.hip)
+void temp_launch(hipStream_t st, int kind, const FdmPairDev *pdev, const int2 *idev, unsigned nitems, double beta,
+                 double Z, double *priv);
+
 // One step of a spectral path: uploads [GEMM descriptors | pair descriptors | GEMM items | element
 // items] as one blob, runs the products on the matrix cores, then kernels(pairs_dev, items_dev) with
 // the element items as int2 (pair, tile). Timing: phase 0 the products, phase 1 the rest.
