@@ -1232,19 +1232,117 @@ def bench_dm(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_nrg(args):
+    """A whole NRG calculation at a configuration's parameter point (workloads.NRG_CONFIGS: c1 the
+    single-channel DM-NRG run, c5 the two-channel keep-4000 FDM run), through libnrgrun.so: every step
+    assembled, diagonalised, recalculated and measured on the GPU. --mode fp64 | mixed | both (the
+    eigensolver form; both: the two runs' output files are compared with each other)."""
+    import shutil
+    import tempfile
+    import torch
+    from nrgljubljana_amd import run_nrg, workloads
+    from tests.compare_outputs import compare_files
+    forms = ["fp64", "mixed"] if args.mode == "both" else [args.mode]
+    if any(f not in ("fp64", "mixed") for f in forms):
+        sys.exit("bench.py: --workload nrg takes --mode fp64, mixed or both")
+    base = tempfile.mkdtemp(prefix="nrg_bench_")
+    # warm-up: a three-step run loads every kernel
+    wd = os.path.join(base, "warm")
+    workloads.nrg_config(args.config, wd, keep=min(args.keep or 100, 100), nmax=3)
+    run_nrg(wd, log_path=os.path.join(wd, "log"))
+    runs = {}
+    for form in forms:
+        d = os.path.join(base, form)
+        N = workloads.nrg_config(args.config, d, diag="default" if form == "fp64" else "mixed", keep=args.keep,
+                                 nmax=args.chain)
+        log(f"{args.config} {form}: chain length {N}, running")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_nrg(d, log_path=os.path.join(d, "log"))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        with open(os.path.join(d, "nrg_timing.json")) as F:
+            tm = json.load(F)
+        n3 = sum(sum(float(n) ** 3 for n in step) for step in tm["blocks"])
+        eig_flops = 10.0 / 3.0 * n3
+        steps = len(tm["nrg_steps"]) + len(tm["dm_steps"])
+        runs[form] = dict(dir=d, wall=wall, N=N, timing=tm, eig_flops=eig_flops, steps=steps)
+        log(f"{args.config} {form}: {wall:.2f} s, {len(tm['nrg_steps'])} NRG + {len(tm['dm_steps'])} DM steps, "
+            f"sections " + ", ".join(f"{k} {v:.2f} s" for k, v in sorted(tm["sections"].items())) +
+            f", eigensolver {eig_flops / tm['sections'].get('diag', 1e-30) / 1e9:.1f} GFLOP/s, largest block "
+            f"{max(max(b) for b in tm['blocks'] if b)}")
+    cmp = None
+    if len(forms) == 2:
+        cmp = {}
+        for f in sorted(os.listdir(runs["fp64"]["dir"])):
+            if f in ("param", "data", "log", "nrg_timing.json", "DONE"):
+                continue
+            ok, err, _ = compare_files(os.path.join(runs["mixed"]["dir"], f), os.path.join(runs["fp64"]["dir"], f))
+            cmp[f] = err
+        log("mixed against fp64, max |diff| per file: " + ", ".join(f"{k} {v:.1e}" for k, v in cmp.items()))
+    main_form = forms[-1] if args.mode != "both" else "mixed"
+    r = runs[main_form]
+    tm = r["timing"]
+    cpu = None
+    if not args.no_cpu:
+        cpu = nrg_cpu_sample(tm["blocks"])
+    out = {"metric": "NRG step wall-time (end-to-end run)", "value": round(1e3 * r["wall"] / r["steps"], 3),
+           "unit": "ms/step", "n_gpus": 1, "steps": r["steps"], "warmup": 1, "ms_per_step": round(1e3 * r["wall"] / r["steps"], 3),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if main_form == "fp64" else "f32+f64", "data": "synthetic",
+           "config": {"workload": f"{args.config}: " + ", ".join(f"{k}={v}" for k, v in workloads.NRG_CONFIGS[args.config].items()),
+                      "chain_length": r["N"], "keep": args.keep or workloads.NRG_CONFIGS[args.config]["keep"],
+                      "form": main_form},
+           "wall_s": {k: round(v["wall"], 3) for k, v in runs.items()},
+           "sections_s": {k: {s: round(t, 3) for s, t in v["timing"]["sections"].items()} for k, v in runs.items()},
+           "eig_gflops": {k: round(v["eig_flops"] / v["timing"]["sections"].get("diag", 1e-30) / 1e9, 1)
+                          for k, v in runs.items()},
+           "largest_block": max(max(b) for b in tm["blocks"] if b),
+           "blocks_per_step_max": max(len(b) for b in tm["blocks"]),
+           "mixed_vs_fp64_max_abs_diff": cmp, "cpu_baseline": cpu}
+    shutil.rmtree(base, ignore_errors=True)
+    print(json.dumps(out), flush=True)
+
+
+def nrg_cpu_sample(blocks, budget_s=20.0):
+    """The oracle's dsyev (OpenMP over blocks, largest first) on synthetic symmetric matrices with the
+    block orders of the run's most expensive step, largest first, until the budget is spent."""
+    import oracle
+    step = max(blocks, key=lambda b: sum(float(n) ** 3 for n in b))
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    rng = np.random.default_rng(5)
+    done, flops, t_tot = 0, 0.0, 0.0
+    for n in sorted(step, reverse=True):
+        if t_tot > budget_s:
+            break
+        a = np.triu(rng.standard_normal((n, n)))
+        t0 = time.perf_counter()
+        oracle.syev_batched(np.array([n]), np.array([0]), a.ravel(), np.array([0]), threads=1, blas_threads=cores)
+        t_tot += time.perf_counter() - t0
+        flops += 10.0 / 3.0 * n ** 3
+        done += 1
+    return {"value": round(flops / t_tot / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of the {len(step)} blocks of the run's most expensive step (largest first, orders "
+                      f"{max(step)} down), oracle dsyev with {cores} BLAS threads, {t_tot:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="eig",
-                    choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "cfs", "ft", "mats", "temp", "ham", "step"])
+                    choices=["eig", "recalc", "fdm", "broaden", "dm", "dmnrg", "cfs", "ft", "mats", "temp", "ham", "step",
+                             "nrg"])
     ap.add_argument("--mode", default="fp64", help="eig: fp64 | mixed; recalc: fp64 | ozaki")
     ap.add_argument("--nblk", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=16, help="eig: smallest block order")
     ap.add_argument("--nmax", type=int, default=1024, help="eig: largest block order")
     ap.add_argument("--npairs", type=int, default=48)
     ap.add_argument("--mats", type=int, default=100, help="mats: Matsubara frequencies")
+    ap.add_argument("--config", default="c5", choices=["c1", "c5"], help="nrg: configuration")
+    ap.add_argument("--keep", type=int, default=None, help="nrg: override the configuration's keep")
+    ap.add_argument("--chain", type=int, default=None, help="nrg: override the chain length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -1254,7 +1352,7 @@ def main():
         sys.exit(2)
     {"eig": bench_eig, "recalc": bench_recalc, "fdm": bench_fdm, "broaden": bench_broaden,
      "dm": bench_dm, "dmnrg": bench_dmnrg, "cfs": bench_cfs, "ft": bench_ft, "mats": bench_mats,
-     "temp": bench_temp, "ham": bench_ham, "step": bench_step}[args.workload](args)
+     "temp": bench_temp, "ham": bench_ham, "step": bench_step, "nrg": bench_nrg}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -175,6 +175,11 @@ class Context:
         offH_a, p_oh = _i64(off_H)
         offw_a, p_ow = _i64(off_w)
         nblk = len(sizes_a)
+        # the library sees raw pointers: check here that the tensors hold every block it will touch
+        for b in range(nblk):
+            n = int(sizes_a[b])
+            if n > 0 and (offH_a[b] < 0 or offH_a[b] + n * n > H.numel() or offw_a[b] < 0 or offw_a[b] + n > w.numel()):
+                raise ValueError(f"syev_batched: block {b} of order {n} does not fit the H / w tensors")
         info = (ctypes.c_int * max(nblk, 1))()
         r = lib().nrg_syev_batched(self._h, nblk, p_n, p_oh, ctypes.c_void_p(H.data_ptr()), p_ow,
                                    ctypes.c_void_p(w.data_ptr()), int(mode), info, _stream_ptr(stream))

@@ -11,7 +11,7 @@
 namespace nrg {
 
 constexpr int WAVE = 64;
-constexpr int NMAX = 4096;   // largest block order accepted by the eigensolver
+constexpr int NMAX = 8192;   // largest block order accepted by the eigensolver
 
 // ---------------------------------------------------------------- error handling
 void set_error(const std::string &msg);
@@ -119,6 +119,7 @@ struct nrg_ctx {
   nrg::DevBuf ws2;       // second workspace (typed arenas of the mixed form)
   nrg::DevBuf desc;      // descriptor tables
   nrg::DevBuf flags;     // per-block status words
+  nrg::DevBuf dcbuf;     // deflation arrays of divide-and-conquer merges too large for LDS
   nrg::PinnedBuf hdesc;  // pinned staging of descriptor tables
   nrg::PinnedBuf hflags;
   nrg::KTimer kt;

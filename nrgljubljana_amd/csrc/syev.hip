@@ -797,10 +797,15 @@ constexpr int NT_DEF = 512;
 // dynamic LDS of k_dc_deflate for merges of order <= mc: d and z (doubles), predecessor (int) and
 // flags (byte) per index
 inline int dc_deflate_lds(int mc) { return (int)(2 * sizeof(double) * (size_t)mc + 5 * (size_t)mc + 64); }
+// dynamic LDS a merge kernel may ask for: the 160 KiB of a workgroup less room for static arrays
+constexpr int DYN_LDS_MAX = 160 * 1024 - 8 * 1024;
 
+// gscr != nullptr: merges too large for LDS keep d, z, predecessors and flags at gscr + merge * gstride
+// (global memory, one region per workgroup) instead of in dynamic LDS
 __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const int2 *items, int depth,
-                                                       MergeState *mst) {
-  extern __shared__ double dyn[];
+                                                       MergeState *mst, double *gscr, int64_t gstride) {
+  extern __shared__ double dyn_lds[];
+  double *dyn = gscr ? gscr + (size_t)blockIdx.x * gstride : dyn_lds;
   __shared__ double red[NT_DEF / WAVE];
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
@@ -969,6 +974,9 @@ __global__ __launch_bounds__(256) void k_dc_roots(SyevArgs<double> a, const int2
 
 // zhat_i^2 = prod_j (lambda_j - d_i) / prod_{j != i} (d_j - d_i), formed from d_i - lambda_j computed
 // relative to each root's origin
+// LDS: stage the merge's d, origins and offsets in dynamic LDS (merges that fit), else read them from
+// the merge's global arrays
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_dc_zhat(SyevArgs<double> a, const int2 *mitems, const int2 *citems, int depth,
                                                  const MergeState *mst) {
   extern __shared__ double dyn[];
@@ -978,14 +986,20 @@ __global__ __launch_bounds__(256) void k_dc_zhat(SyevArgs<double> a, const int2 
   const NodeGeo g = node_geo((int)bd.n, depth, it.y);
   const int k = mst[ci.x].k;
   if (ci.y * 256 >= k) return;
-  double *dl = dyn, *ol = dyn + g.m, *tl = dyn + 2 * g.m;
   const double *dnd = dsa(a, bd, DS_DND, g.a0), *dorg = dsa(a, bd, DS_DORG, g.a0), *rt = dsa(a, bd, DS_RT, g.a0);
-  for (int i = threadIdx.x; i < k; i += 256) {
-    dl[i] = dnd[i];
-    ol[i] = dorg[i];
-    tl[i] = rt[i];
+  const double *dl = dnd, *ol = dorg, *tl = rt;
+  if constexpr (LDS) {
+    double *dls = dyn, *ols = dyn + g.m, *tls = dyn + 2 * g.m;
+    for (int i = threadIdx.x; i < k; i += 256) {
+      dls[i] = dnd[i];
+      ols[i] = dorg[i];
+      tls[i] = rt[i];
+    }
+    __syncthreads();
+    dl = dls;
+    ol = ols;
+    tl = tls;
   }
-  __syncthreads();
   const int i = ci.y * 256 + threadIdx.x;
   if (i < k) {
     const double di = dl[i];
@@ -999,6 +1013,8 @@ __global__ __launch_bounds__(256) void k_dc_zhat(SyevArgs<double> a, const int2 
 }
 
 // eigenvector norms of the roots, and ranks of all m eigenvalues of the merge
+// LDS as for k_dc_zhat; without it the eigenvalue of index v is formed where it is read
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_dc_normrank(SyevArgs<double> a, const int2 *mitems, const int2 *citems,
                                                      int depth, const MergeState *mst) {
   extern __shared__ double dyn[];
@@ -1008,27 +1024,33 @@ __global__ __launch_bounds__(256) void k_dc_normrank(SyevArgs<double> a, const i
   const NodeGeo g = node_geo((int)bd.n, depth, it.y);
   const int m = g.m, k = mst[ci.x].k;
   if (ci.y * 256 >= m) return;
-  double *dl = dyn, *zl = dyn + m, *vl = dyn + 2 * m;
   const double *dnd = dsa(a, bd, DS_DND, g.a0), *zhat = dsa(a, bd, DS_ZHAT, g.a0);
   const double *dorg = dsa(a, bd, DS_DORG, g.a0), *rt = dsa(a, bd, DS_RT, g.a0);
   const double *ds = dsa(a, bd, DS_DS, g.a0);
   const int *cpos = isa(a, bd, IS_CPOS, g.a0);
-  for (int i = threadIdx.x; i < m; i += 256) {
-    if (i < k) {
-      dl[i] = dnd[i];
-      zl[i] = zhat[i];
-      vl[i] = dorg[i] + rt[i];
-    } else {
-      vl[i] = ds[clampi(cpos[i], m)];
+  auto value = [&](int i) { return i < k ? dorg[i] + rt[i] : ds[clampi(cpos[i], m)]; };
+  const double *dl = dnd, *zl = zhat;
+  double *vl = nullptr;
+  if constexpr (LDS) {
+    double *dls = dyn, *zls = dyn + m;
+    vl = dyn + 2 * m;
+    for (int i = threadIdx.x; i < m; i += 256) {
+      if (i < k) {
+        dls[i] = dnd[i];
+        zls[i] = zhat[i];
+      }
+      vl[i] = value(i);
     }
+    __syncthreads();
+    dl = dls;
+    zl = zls;
   }
-  __syncthreads();
   const int u = ci.y * 256 + threadIdx.x;
   if (u < m) {
-    const double x = vl[u], kx = okey(x);
+    const double x = LDS ? vl[u] : value(u), kx = okey(x);
     int r = 0;
     for (int v = 0; v < m; v++) {
-      const double kv = okey(vl[v]);
+      const double kv = okey(LDS ? vl[v] : value(v));
       r += (kv < kx) || (kv == kx && v < u);
     }
     isa(a, bd, IS_COLSRC, g.a0)[r] = u;
@@ -1402,13 +1424,13 @@ __global__ __launch_bounds__(256) void k_ref_clusters(RefArgs r, const int *blis
   __shared__ double ev[CMAX];
   __shared__ int perm[CMAX];
   __shared__ double cs[2];
-  __shared__ double lts[NMAX];
-  for (int i = threadIdx.x; i < n; i += 256) lts[i] = lt[i];
-  __syncthreads();
+  // The scan of a cluster starting at `start` reads lt[start, end]; the cluster's new values are
+  // written to lt[start, end) only after its scan, and the next scan starts at `end`: every value
+  // read is still the estimate from before this kernel.
   int start = 0;
   while (start < n) {
     int end = start + 1;
-    while (end < n && end - start < CMAX && fabs(lts[end] - lts[end - 1]) <= delta) end++;
+    while (end < n && end - start < CMAX && fabs(lt[end] - lt[end - 1]) <= delta) end++;
     const int c = end - start;
     if (c >= 2) {
       // B = S_CC + (R_CC S_CC + S_CC R_CC) / 2
@@ -1483,7 +1505,7 @@ __global__ __launch_bounds__(256) void k_ref_clusters(RefArgs r, const int *blis
         }
       }
       __syncthreads();
-      for (int i = threadIdx.x; i < c; i += 256) lt[start + i] = ev[perm[i]];   // lts keeps the scan order
+      for (int i = threadIdx.x; i < c; i += 256) lt[start + i] = ev[perm[i]];
       // X_C <- X_C Q(:, perm)
       for (int row = threadIdx.x; row < n; row += 256) {
         double xr[CMAX];
@@ -1551,7 +1573,7 @@ int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, dou
   if (nblk == 0) return NRG_OK;
   for (int b = 0; b < nblk; b++)
     if (n[b] < 1 || n[b] > NMAX) {
-      set_error("nrg_syev_batched: block order out of range [1, 4096]");
+      set_error("nrg_syev_batched: block order out of range [1, " + std::to_string(NMAX) + "]");
       return NRG_E_ARG;
     }
   if (red_groups() > 1)
@@ -2031,6 +2053,9 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
       if (c.rows <= NT_PANEL)
         NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, sg, aT,
                                                             I + c.prep_off, c.jl));
+      else if (c.rows <= 4 * NT_PANEL)
+        NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 4>), grid(c.prep_cnt), dim3(NT_PANEL), 0, sg, aT,
+                                                            I + c.prep_off, c.jl));
       else
         NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, NMAX / NT_PANEL>), grid(c.prep_cnt), dim3(NT_PANEL),
                                                             0, sg, aT, I + c.prep_off, c.jl));
@@ -2057,11 +2082,29 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     int mc = 0;
     for (int depth = 0; depth < pl.Lmax; depth++) mc = std::max(mc, pl.mcap[depth]);
     const int lds3 = (int)(3 * sizeof(double) * (size_t)mc + 16), lds2 = (int)(2 * sizeof(double) * (size_t)mc + 16);
-    (void)hipFuncSetAttribute((const void *)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize, dc_deflate_lds(mc));
+    (void)hipFuncSetAttribute((const void *)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              std::min(dc_deflate_lds(mc), DYN_LDS_MAX));
     (void)hipFuncSetAttribute((const void *)k_dc_roots, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
-    (void)hipFuncSetAttribute((const void *)k_dc_zhat, hipFuncAttributeMaxDynamicSharedMemorySize, lds3);
-    (void)hipFuncSetAttribute((const void *)k_dc_normrank, hipFuncAttributeMaxDynamicSharedMemorySize, lds3);
+    (void)hipFuncSetAttribute((const void *)k_dc_zhat<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              std::min(lds3, DYN_LDS_MAX));
+    (void)hipFuncSetAttribute((const void *)k_dc_normrank<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              std::min(lds3, DYN_LDS_MAX));
     (void)hipFuncSetAttribute((const void *)k_dc_fill, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
+  }
+  // merges whose deflation does not fit the LDS keep its arrays in global memory, one region each
+  double *dc_gscr = nullptr;
+  int64_t dc_gstride = 0;
+  {
+    size_t need = 0;
+    for (int depth = 0; depth < pl.Lmax; depth++)
+      if (pl.mrg_cnt[depth] && dc_deflate_lds(pl.mcap[depth]) > DYN_LDS_MAX) {
+        dc_gstride = std::max<int64_t>(dc_gstride, (dc_deflate_lds(pl.mcap[depth]) + 7) / 8);
+        need = std::max(need, (size_t)pl.mrg_cnt[depth]);
+      }
+    if (need) {
+      if (int r = ctx->dcbuf.ensure(sizeof(double) * need * (size_t)dc_gstride)) return r;
+      dc_gscr = ctx->dcbuf.as<double>();
+    }
   }
   NRG_TIMED(ctx, st, "k_dc_prep", hipLaunchKernelGGL(k_dc_prep, grid(pl.bd.size()), dim3(256), 0, st, aD, I + pl.all_off));
   for (int depth = pl.Lmax - 1; depth >= 0; depth--) {
@@ -2071,15 +2114,23 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     const int2 *mi = (const int2 *)(I + pl.mrg_off[depth]);
     const int2 *ch = (const int2 *)(I + pl.ch_off[depth]);
     const int2 *fc = (const int2 *)(I + pl.fc_off[depth]);
+    const bool dg = dc_deflate_lds(mc) > DYN_LDS_MAX, lg = (int)lds3 > DYN_LDS_MAX;
     NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate, grid(pl.mrg_cnt[depth]), dim3(NT_DEF),
-                                                          dc_deflate_lds(mc), st, aD,
-                                                          mi, depth, L.mst));
+                                                          dg ? 0 : dc_deflate_lds(mc), st, aD, mi, depth, L.mst,
+                                                          dg ? dc_gscr : nullptr, dc_gstride));
     NRG_TIMED(ctx, st, "k_dc_roots", hipLaunchKernelGGL(k_dc_roots, grid(pl.rc_cnt[depth]), dim3(256), lds2, st, aD, mi,
                                                         (const int2 *)(I + pl.rc_off[depth]), depth, (const MergeState *)L.mst));
-    NRG_TIMED(ctx, st, "k_dc_zhat", hipLaunchKernelGGL(k_dc_zhat, grid(pl.ch_cnt[depth]), dim3(256), lds3, st, aD, mi, ch,
-                                                       depth, (const MergeState *)L.mst));
-    NRG_TIMED(ctx, st, "k_dc_normrank", hipLaunchKernelGGL(k_dc_normrank, grid(pl.ch_cnt[depth]), dim3(256), lds3, st, aD,
-                                                           mi, ch, depth, (const MergeState *)L.mst));
+    if (lg) {
+      NRG_TIMED(ctx, st, "k_dc_zhat", hipLaunchKernelGGL(k_dc_zhat<false>, grid(pl.ch_cnt[depth]), dim3(256), 0, st, aD,
+                                                         mi, ch, depth, (const MergeState *)L.mst));
+      NRG_TIMED(ctx, st, "k_dc_normrank", hipLaunchKernelGGL(k_dc_normrank<false>, grid(pl.ch_cnt[depth]), dim3(256), 0,
+                                                             st, aD, mi, ch, depth, (const MergeState *)L.mst));
+    } else {
+      NRG_TIMED(ctx, st, "k_dc_zhat", hipLaunchKernelGGL(k_dc_zhat<true>, grid(pl.ch_cnt[depth]), dim3(256), lds3, st, aD,
+                                                         mi, ch, depth, (const MergeState *)L.mst));
+      NRG_TIMED(ctx, st, "k_dc_normrank", hipLaunchKernelGGL(k_dc_normrank<true>, grid(pl.ch_cnt[depth]), dim3(256), lds3,
+                                                             st, aD, mi, ch, depth, (const MergeState *)L.mst));
+    }
     NRG_TIMED(ctx, st, "k_dc_fill", hipLaunchKernelGGL(k_dc_fill, grid(pl.fc_cnt[depth]), dim3(256), lds2, st, aD, mi, fc,
                                                        depth, (const MergeState *)L.mst));
     if (pl.gemm_cnt[depth]) gemm_launch(ctx, st, L.dG, I + pl.gemm_off[depth], H + pl.gemm_off[depth], pl.gemm_cnt[depth]);

@@ -170,6 +170,7 @@ struct Spectrum {
 struct Timing {
   std::map<std::string, double> s;   // seconds per section
   std::vector<std::pair<size_t, double>> nrg_steps, dm_steps;   // wall time of every step
+  std::vector<std::vector<size_t>> blocks;   // block orders diagonalised at every forward step
 };
 
 class NRGCalculation {
@@ -375,6 +376,15 @@ class NRGCalculation {
         }
         DiagParams DP(P_.diag == "default" ? "default" : P_.diag, diagratio);
         DP.check = false;
+        std::vector<size_t> sizes;
+        double n3 = 0;
+        for (const auto &t : tasks) {
+          sizes.push_back(t.first);
+          n3 += std::pow((double)t.first, 3);
+        }
+        timing_.blocks.push_back(sizes);
+        log_ << "N=" << step.ndx() << " blocks " << tasks.size() << " max " << (tasks.empty() ? 0 : tasks.front().first)
+             << " sum n^3 " << n3 << "\n";
         diag = diagonalisations(H, DP, dev_, step.last());
       } else {
         diag = steps_.load(step.ndx());

@@ -35,6 +35,12 @@ void write_timing(const std::string &dir, const NRG::Timing &t) {
   for (size_t i = 0; i < t.nrg_steps.size(); i++) F << (i ? ", " : "") << t.nrg_steps[i].second;
   F << "], \"dm_steps\": [";
   for (size_t i = 0; i < t.dm_steps.size(); i++) F << (i ? ", " : "") << t.dm_steps[i].second;
+  F << "], \"blocks\": [";
+  for (size_t i = 0; i < t.blocks.size(); i++) {
+    F << (i ? ", " : "") << "[";
+    for (size_t k = 0; k < t.blocks[i].size(); k++) F << (k ? ", " : "") << t.blocks[i][k];
+    F << "]";
+  }
   F << "]}\n";
 }
 }  // namespace

@@ -6,6 +6,8 @@ Haar-random Q (QR of a Gaussian matrix with the signs of diag(R) fixed), sizes l
 1e-10 and one exactly degenerate pair. Blocks are packed into one arena, row-major with the upper
 triangle filled, as diagonalise() receives them (c++/diag.hpp:265).
 """
+import os
+
 import numpy as np
 
 SEED = 20261203
@@ -227,3 +229,46 @@ def dm_flops(step):
         for ch in sub["children"]:
             s += 2.0 * r * ch["nromega"] ** 2 + 2.0 * r * r * ch["nromega"]
     return s
+
+
+# ---- end-to-end NRG runs at the configurations' parameter points
+NRG_CONFIGS = {
+    # single-channel Anderson model, QS, Lambda = 2, U = 0.1, eps_d = -U/2, Gamma = 0.01, T = 1e-8,
+    # keep <= 1000 multiplets with cut-off energy 10, DM-NRG spectral function of d
+    "c1": dict(channels=1, U=0.1, Gamma=0.01, Lambda=2.0, T=1e-8, keep=1000, keepenergy=10.0,
+               algo="dmnrg=true", ops="n_d A_d"),
+    # two-channel Anderson model, QS, Lambda = 3, U = 0.2, Gamma_1 = Gamma_2 = 0.02, T = 1e-10, keep 4000
+    # multiplets, FDM spectral function of d and FDM thermodynamics
+    "c5": dict(channels=2, U=0.2, Gamma=0.02, Lambda=3.0, T=1e-10, keep=4000, keepenergy=-1.0,
+               algo="fdm=true", ops="n_d A_d"),
+}
+
+
+def nrg_chain_length(Lam, T):
+    """Chain length of a run down to temperature T: the first N with SCALE(N) <= T (discretisation Y)."""
+    from .initial import scale
+    N = 1
+    while scale(N, Lam, "Y") > T:
+        N += 1
+    return N
+
+
+def nrg_config(name, directory, diag="default", keep=None, nmax=None):
+    """Writes `param` and `data` of configuration `name` (NRG_CONFIGS) into `directory`: flat-band
+    Wilson chain (discretisation Y), eps_d = -U/2 (delta = 0), the initial cluster from
+    initial.siam_qs. `keep` and `nmax` override the configuration (shorter runs for tests).
+    Returns the chain length."""
+    from . import initial
+    c = NRG_CONFIGS[name]
+    N = nmax or nrg_chain_length(c["Lambda"], c["T"])
+    k = keep or c["keep"]
+    os.makedirs(directory, exist_ok=True)
+    cl = initial.siam_qs(c["U"], 0.0, [c["Gamma"]] * c["channels"], c["Lambda"], "Y")
+    xi = [initial.wilson_xi(c["Lambda"], N)] * c["channels"]
+    zeta = [np.zeros(N + 1)] * c["channels"]
+    initial.write_data(os.path.join(directory, "data"), cl, xi, zeta, ops=tuple(c["ops"].split()))
+    with open(os.path.join(directory, "param"), "w") as F:
+        F.write("[param]\nsymtype=QS\ndiscretization=Y\n"
+                f"Lambda={c['Lambda']}\nT={c['T']}\nkeep={k}\nkeepenergy={c['keepenergy']}\n"
+                f"ops={c['ops']}\nspecd=A_d-A_d\n{c['algo']}\ndiag={diag}\n")
+    return N
