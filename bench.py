@@ -1239,9 +1239,11 @@ def bench_nrg(args):
     eigensolver form; both: the two runs' output files are compared with each other)."""
     import shutil
     import tempfile
-    import torch
     from nrgljubljana_amd import run_nrg, workloads
     from tests.compare_outputs import compare_files
+    # torch is not imported here: nrg_run_dir is synchronous, and a process that imports torch loads the
+    # HIP runtime bundled with torch in place of the system one (same SONAME), under which rocprofv3's
+    # kernel tracing was seen to corrupt this workload's results
     forms = ["fp64", "mixed"] if args.mode == "both" else [args.mode]
     if any(f not in ("fp64", "mixed") for f in forms):
         sys.exit("bench.py: --workload nrg takes --mode fp64, mixed or both")
@@ -1256,10 +1258,8 @@ def bench_nrg(args):
         N = workloads.nrg_config(args.config, d, diag="default" if form == "fp64" else "mixed", keep=args.keep,
                                  nmax=args.chain)
         log(f"{args.config} {form}: chain length {N}, running")
-        torch.cuda.synchronize()
         t0 = time.perf_counter()
         run_nrg(d, log_path=os.path.join(d, "log"))
-        torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         with open(os.path.join(d, "nrg_timing.json")) as F:
             tm = json.load(F)
@@ -1271,6 +1271,13 @@ def bench_nrg(args):
             f"sections " + ", ".join(f"{k} {v:.2f} s" for k, v in sorted(tm["sections"].items())) +
             f", eigensolver {eig_flops / tm['sections'].get('diag', 1e-30) / 1e9:.1f} GFLOP/s, largest block "
             f"{max(max(b) for b in tm['blocks'] if b)}")
+    if args.save:
+        for form, r in runs.items():
+            dst = os.path.join(args.save, form)
+            os.makedirs(dst, exist_ok=True)
+            for f in os.listdir(r["dir"]):
+                if f != "data" and os.path.isfile(os.path.join(r["dir"], f)):
+                    shutil.copy(os.path.join(r["dir"], f), dst)
     cmp = None
     if len(forms) == 2:
         cmp = {}
@@ -1345,6 +1352,7 @@ def main():
     ap.add_argument("--chain", type=int, default=None, help="nrg: override the chain length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--save", default=None, help="nrg: copy each form's output files to SAVE/<form>")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: runs on one GPU only (sharding the blocks of a step across GPUs is a design, not "

@@ -116,6 +116,23 @@ int nrg_syev_batched(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *of
                      const int64_t *off_w, double *w_arena, int mode, int *info, void *stream);
 
 /*
+ * check_diag of a whole step on the device: replaces the check that follows every diagonalise
+ * (c++/diag.hpp:290 -> check_diag, c++/numerics.hpp:340-404: eigenvalues finite, eigenvector rows
+ * orthonormal by is_unitary's scalar loops). For each block b the first rows[b] rows of the row-major
+ * n[b] x n[b] eigenvector matrix at V_arena + off_V[b] (as nrg_syev_batched leaves them) must satisfy
+ * |<v_r, v_r> - 1| <= eps_norm and |<v_r, v_s>| <= eps_orth (r != s; the reference uses 1e-12 and
+ * 1e-10), and w_arena[off_w[b] .. off_w[b] + rows[b]) must be finite. The Gram matrices of all blocks
+ * are one grouped FP64 matrix-core product.
+ *   dev_norm[b], dev_orth[b] (host, may be NULL): max |<v_r, v_r> - 1| and max_{r != s} |<v_r, v_s>|
+ *                            (+inf where a product is not finite)
+ *   info[b]      (host, may be NULL): 0 ok, 1 normalisation, 2 orthogonality, 3 non-finite eigenvalue
+ * Returns NRG_E_NOCONV if any block fails; the call is synchronous on `stream`.
+ */
+int nrg_check_diag_batched(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *rows, const int64_t *off_V,
+                           const double *V_arena, const int64_t *off_w, const double *w_arena, double eps_norm,
+                           double eps_orth, double *dev_norm, double *dev_orth, int *info, void *stream);
+
+/*
  * One term of a grouped triple product, in the reference's row-major layout:
  *   C[m x n] += f * opA[m x k] * O[k x l] * opB[l x n]
  * with

@@ -18,6 +18,7 @@ NRG_GEMM_FP64 = 0
 NRG_GEMM_OZAKI = 1
 
 EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
+            "nrg_check_diag_batched",
             "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_cfs_bin", "nrg_ft_bin", "nrg_fdm_mats", "nrg_dmnrg_mats",
             "nrg_ft_mats", "nrg_temp_sum", "nrg_temp_sum_noalloc", "nrg_broaden", "nrg_hamiltonian",
             "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
@@ -80,6 +81,10 @@ def lib():
     L.nrg_syev_batched.argtypes = [vp, ctypes.c_int, i64p, i64p, vp, i64p, vp, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int), vp]
     L.nrg_syev_batched.restype = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.nrg_check_diag_batched.argtypes = [vp, ctypes.c_int, i64p, i64p, i64p, vp, i64p, vp, ctypes.c_double,
+                                         ctypes.c_double, dp, dp, ctypes.POINTER(ctypes.c_int), vp]
+    L.nrg_check_diag_batched.restype = ctypes.c_int
     L.nrg_gemm3_grouped.argtypes = [vp, ctypes.POINTER(Gemm3Term), ctypes.c_int, ctypes.c_int, vp]
     L.nrg_gemm3_grouped.restype = ctypes.c_int
     L.nrg_fdm_bin.argtypes = [vp, ctypes.POINTER(FdmPair), ctypes.c_int, ctypes.POINTER(BinGrid), ctypes.c_double,
@@ -187,6 +192,28 @@ class Context:
         if r != 0 and check:
             raise NrgError(r, "nrg_syev_batched")
         return info_np
+
+    def check_diag_batched(self, V, sizes, rows, off_V, w, off_w, eps_norm=1e-12, eps_orth=1e-10, stream=None):
+        """nrg_check_diag_batched on the eigenvector arena V and eigenvalue arena w (flat float64 device
+        tensors). Returns (info, dev_norm, dev_orth) per block; never raises on a failed check."""
+        sizes_a, p_n = _i64(sizes)
+        rows_a, p_r = _i64(rows)
+        offV_a, p_ov = _i64(off_V)
+        offw_a, p_ow = _i64(off_w)
+        nblk = len(sizes_a)
+        for b in range(nblk):
+            n, m = int(sizes_a[b]), int(rows_a[b])
+            if m > 0 and (offV_a[b] < 0 or offV_a[b] + m * n > V.numel() or offw_a[b] < 0 or offw_a[b] + m > w.numel()):
+                raise ValueError(f"check_diag_batched: block {b} does not fit the V / w tensors")
+        info = (ctypes.c_int * max(nblk, 1))()
+        dn = (ctypes.c_double * max(nblk, 1))()
+        do = (ctypes.c_double * max(nblk, 1))()
+        r = lib().nrg_check_diag_batched(self._h, nblk, p_n, p_r, p_ov, ctypes.c_void_p(V.data_ptr()), p_ow,
+                                         ctypes.c_void_p(w.data_ptr()), float(eps_norm), float(eps_orth), dn, do,
+                                         info, _stream_ptr(stream))
+        if r not in (0, 3):
+            raise NrgError(r, "nrg_check_diag_batched")
+        return (np.array(info[:nblk], dtype=np.int32), np.array(dn[:nblk]), np.array(do[:nblk]))
 
     # ------------------------------------------------------------------ grouped triple products
     def gemm3_grouped(self, terms, precision=NRG_GEMM_FP64, stream=None):

@@ -2168,6 +2168,9 @@ struct Carver {
 };
 
 static int finish_flags(nrg_ctx *ctx, int nblk, int *info, hipStream_t st) {
+  // the kernels complete before the copy is issued (a DMA copy was seen to overtake kernels queued
+  // before it under rocprofv3's kernel tracing)
+  NRG_HIP(hipStreamSynchronize(st));
   NRG_HIP(hipMemcpyAsync(ctx->hflags.p, ctx->flags.p, sizeof(int) * nblk, hipMemcpyDeviceToHost, st));
   NRG_HIP(hipStreamSynchronize(st));
   const int *f = (const int *)ctx->hflags.p;

@@ -17,6 +17,19 @@ namespace NRG {
 inline void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+// Blocking copies of the host layer. Each first waits for the null stream, so that the copy is issued only
+// after every fill, copy and library kernel queued before it has completed. Stream order alone should
+// give this; under rocprofv3's kernel tracing a strided copy (run on a DMA engine) was seen to overtake
+// the fill kernel queued before it, so the order is made explicit here.
+inline void dcopy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+  hip_check(hipMemcpy(dst, src, bytes, kind), "hipMemcpy");
+}
+inline void dcopy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                    hipMemcpyKind kind) {
+  hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+  hip_check(hipMemcpy2D(dst, dpitch, src, spitch, width, height, kind), "hipMemcpy2D");
+}
 inline void nrg_check(int status, const char *what) {
   if (status != NRG_OK) throw std::runtime_error(std::string(what) + " failed: " + nrg_last_error());
 }
@@ -97,26 +110,23 @@ struct DView {
 inline Matrix download(const DView &v) {
   Matrix m(v.rows, v.cols);
   if (!v.empty())
-    hip_check(hipMemcpy2D(m.data(), v.cols * sizeof(double), v.p, v.ld * sizeof(double), v.cols * sizeof(double),
-                          v.rows, hipMemcpyDeviceToHost),
-              "hipMemcpy2D");
+    dcopy2d(m.data(), v.cols * sizeof(double), v.p, v.ld * sizeof(double), v.cols * sizeof(double),
+                          v.rows, hipMemcpyDeviceToHost);
   return m;
 }
 inline void upload(const DView &v, const Matrix &m) {
   NRG_ASSERT(m.size1() == v.rows && m.size2() == v.cols);
   if (!v.empty())
-    hip_check(hipMemcpy2D(v.p, v.ld * sizeof(double), m.data(), v.cols * sizeof(double), v.cols * sizeof(double),
-                          v.rows, hipMemcpyHostToDevice),
-              "hipMemcpy2D");
+    dcopy2d(v.p, v.ld * sizeof(double), m.data(), v.cols * sizeof(double), v.cols * sizeof(double),
+                          v.rows, hipMemcpyHostToDevice);
 }
 // sum of the diagonal of a square device matrix
 inline double trace_real(const DView &v) {
   const size_t n = std::min(v.rows, v.cols);
   if (!n) return 0.0;
   std::vector<double> d(n);
-  hip_check(hipMemcpy2D(d.data(), sizeof(double), v.p, (v.ld + 1) * sizeof(double), sizeof(double), n,
-                        hipMemcpyDeviceToHost),
-            "hipMemcpy2D");
+  dcopy2d(d.data(), sizeof(double), v.p, (v.ld + 1) * sizeof(double), sizeof(double), n,
+                        hipMemcpyDeviceToHost);
   double t = 0;
   for (double x : d) t += x;
   return t;

@@ -76,9 +76,30 @@ struct BatchResult {
   std::vector<int64_t> n, off;
   std::vector<std::vector<double>> val;
 };
-// nrg_syev_batched on blocks already in a device arena (block b: n[b] x n[b] at off[b])
+// check_diag (c++/numerics.hpp:340-404) of the first rows[b] eigenpairs of every block, on the device
+// (nrg_check_diag_batched); thresholds as is_unitary's num_equal: |<v,v> - 1| < 1e-12 (|<v,v>| + 1) and
+// |<v_r,v_s>| < 1e-10
+inline void check_diag_device(Device &dev, const std::vector<int64_t> &n, const std::vector<int64_t> &rows,
+                              const std::vector<int64_t> &off, const double *V, const std::vector<int64_t> &offw,
+                              const double *w) {
+  const size_t nb = n.size();
+  std::vector<double> dn(nb), dof(nb);
+  std::vector<int> info(nb, 0);
+  const int r = nrg_check_diag_batched(dev.ctx(), (int)nb, n.data(), rows.data(), off.data(), V, offw.data(), w, 2e-12,
+                                       1e-10, dn.data(), dof.data(), info.data(), dev.stream());
+  if (r == NRG_OK) return;
+  if (r != NRG_E_NOCONV) nrg_check(r, "nrg_check_diag_batched");
+  for (size_t b = 0; b < nb; b++)
+    if (info[b])
+      throw std::runtime_error("check_diag failed for block " + std::to_string(b) + " (dim " + std::to_string(n[b]) +
+                               "): max |<v,v> - 1| = " + std::to_string(dn[b]) + ", max |<v_r,v_s>| = " +
+                               std::to_string(dof[b]) + (info[b] == 3 ? ", non-finite eigenvalue" : ""));
+}
+
+// nrg_syev_batched on blocks already in a device arena (block b: n[b] x n[b] at off[b]); with
+// check_rows, check_diag_device on the first check_rows[b] eigenpairs of each block
 inline BatchResult syev_arena(Device &dev, std::shared_ptr<DBuf<double>> arena, std::vector<int64_t> n,
-                              std::vector<int64_t> off, int mode) {
+                              std::vector<int64_t> off, int mode, const std::vector<int64_t> *check_rows = nullptr) {
   BatchResult r;
   const size_t nb = n.size();
   r.arena = std::move(arena);
@@ -98,14 +119,16 @@ inline BatchResult syev_arena(Device &dev, std::shared_ptr<DBuf<double>> arena, 
   for (size_t b = 0; b < nb; b++)
     if (info[b] != 0)   // the reference throws on INFO != 0 (c++/diag.hpp:61-82)
       throw std::runtime_error("diagonalisation of block " + std::to_string(b) + " failed, info=" + std::to_string(info[b]));
+  if (check_rows) check_diag_device(dev, r.n, *check_rows, r.off, r.arena->get(), offw, w.get());
   std::vector<double> wh((size_t)totw);
-  hip_check(hipMemcpy(wh.data(), w.get(), sizeof(double) * (size_t)totw, hipMemcpyDeviceToHost), "hipMemcpy");
+  dcopy(wh.data(), w.get(), sizeof(double) * (size_t)totw, hipMemcpyDeviceToHost);
   r.val.resize(nb);
   for (size_t b = 0; b < nb; b++) r.val[b].assign(wh.begin() + offw[b], wh.begin() + offw[b] + r.n[b]);
   return r;
 }
 
-inline BatchResult syev_batch(Device &dev, const std::vector<const Matrix *> &mats, int mode) {
+inline BatchResult syev_batch(Device &dev, const std::vector<const Matrix *> &mats, int mode,
+                              const std::vector<int64_t> *check_rows = nullptr) {
   const size_t nb = mats.size();
   std::vector<int64_t> n(nb), off(nb);
   int64_t tot = 0;
@@ -119,8 +142,8 @@ inline BatchResult syev_batch(Device &dev, const std::vector<const Matrix *> &ma
   for (size_t b = 0; b < nb; b++)
     if (n[b]) std::memcpy(host.data() + off[b], mats[b]->data(), sizeof(double) * (size_t)(n[b] * n[b]));
   auto arena = std::make_shared<DBuf<double>>((size_t)tot + 1);
-  hip_check(hipMemcpy(arena->get(), host.data(), sizeof(double) * (size_t)tot, hipMemcpyHostToDevice), "hipMemcpy");
-  return syev_arena(dev, std::move(arena), std::move(n), std::move(off), mode);
+  dcopy(arena->get(), host.data(), sizeof(double) * (size_t)tot, hipMemcpyHostToDevice);
+  return syev_arena(dev, std::move(arena), std::move(n), std::move(off), mode, check_rows);
 }
 }  // namespace detail
 
@@ -151,13 +174,14 @@ inline DiagInfo diagonalisations(std::vector<std::pair<Invar, Matrix>> &tasks, c
     NRG_ASSERT(is_matrix_upper(t.second));
     mats.push_back(&t.second);
   }
-  auto res = detail::syev_batch(dev, mats, DP.mode());
+  std::vector<int64_t> rows;
+  for (const Matrix *m : mats) rows.push_back((int64_t)DP.nr_wanted(m->size1()));
+  auto res = detail::syev_batch(dev, mats, DP.mode(), DP.check ? &rows : nullptr);
   DiagInfo diag;
   for (size_t b = 0; b < tasks.size(); b++) {
     const size_t n = (size_t)res.n[b], M = DP.nr_wanted(n);
     std::vector<double> val(res.val[b].begin(), res.val[b].begin() + M);
     DView vec{res.arena->get() + res.off[b], M, n, n};
-    if (DP.check && n) check_diag(val, download(vec));
     diag[tasks[b].first] = Eigen(std::move(val), vec, res.arena, last_step);
   }
   tasks.clear();

@@ -5,6 +5,7 @@
 // children, KK and DD sectors - is queued as one rotate term in a Gemm3Batch and the step is evaluated
 // in one grouped call; the density matrices stay in device memory between steps.
 #pragma once
+#include <cstdio>
 #include <functional>
 
 #include "measurements.hpp"
@@ -103,16 +104,25 @@ inline DensMatElements init_rho_FDM(size_t N, const Store &store, const Stats &s
       for (size_t i = ds.min(); i < ds.max(); i++)
         d[i - ds.min()] = std::exp(-ds.eig.values.abs_zero(i) / T) * stats.wn[N] / stats.ZnDNd[N];
       const size_t ld = m.size2();
-      hip_check(hipMemcpy2D(m.data() + ds.min() * (ld + 1), (ld + 1) * sizeof(double), d.data(), sizeof(double),
-                            sizeof(double), d.size(), hipMemcpyHostToDevice),
-                "hipMemcpy2D");
+      dcopy2d(m.data() + ds.min() * (ld + 1), (ld + 1) * sizeof(double), d.data(), sizeof(double),
+                            sizeof(double), d.size(), hipMemcpyHostToDevice);
     }
     rhoFDM[I] = std::move(m);
   }
   if (stats.wn[N] != 0.0) {
     const double tr = rhoFDM.trace(mult);
     const double diff = (tr - stats.wn[N]) / stats.wn[N];
-    if (!num_equal(diff, 0.0, 1e-8)) NRG_ASSERT(stats.wn[N] < 1e-12);
+    if (!num_equal(diff, 0.0, 1e-8) && !(stats.wn[N] < 1e-12)) {
+      size_t nonfinite = 0, states = 0;
+      for (const auto &[I, ds] : store[N])
+        for (size_t i = ds.min(); i < ds.max(); i++, states++) nonfinite += !std::isfinite(ds.eig.values.abs_zero(i));
+      char msg[256];
+      std::snprintf(msg, sizeof msg,
+                    "init_rho_FDM: trace %.17g of the shell-%zu density matrix differs from w_N = %.17g (Z_N^D = %.17g, "
+                    "%zu of %zu discarded energies not finite)",
+                    tr, N, stats.wn[N], stats.ZnDNd[N], nonfinite, states);
+      throw std::runtime_error(msg);
+    }
   }
   return rhoFDM;
 }
@@ -159,7 +169,12 @@ inline void calc_fulldensitymatrix(DensMatElements &rhoFDM, const Store &store, 
     double expected = 0.0;
     for (size_t n = N; n < Nmax; n++) expected += stats.wn[n];
     const double diff = (tr - expected) / expected;
-    if (!num_equal(diff, 0.0)) throw std::runtime_error("calc_fulldensitymatrix: trace check failed at N=" + std::to_string(N));
+    if (!num_equal(diff, 0.0)) {
+      char msg[160];
+      std::snprintf(msg, sizeof msg, "calc_fulldensitymatrix: trace check failed at N=%zu (trace %.17g, expected %.17g)",
+                    N, tr, expected);
+      throw std::runtime_error(msg);
+    }
     if (keep) keep(N - 1, rhoFDMPrev);
     rhoFDM.swap(rhoFDMPrev);
   }

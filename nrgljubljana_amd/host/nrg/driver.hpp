@@ -37,7 +37,7 @@ struct Operators {   // c++/operators.hpp:167-203
 inline DMatrix clone(const DMatrix &m) {
   DMatrix c(m.size1(), m.size2());
   if (m.size1() && m.size2())
-    hip_check(hipMemcpy(c.data(), m.data(), 8 * m.size1() * m.size2(), hipMemcpyDeviceToDevice), "hipMemcpy");
+    dcopy(c.data(), m.data(), 8 * m.size1() * m.size2(), hipMemcpyDeviceToDevice);
   return c;
 }
 inline MatrixElements clone(const MatrixElements &m) {
@@ -69,8 +69,7 @@ inline void trim(MatrixElements &m, const DiagInfo &diag) {
     if (d1 == mat.size1() && d2 == mat.size2()) continue;
     DMatrix t(d1, d2);
     if (d1 && d2)
-      hip_check(hipMemcpy2D(t.data(), 8 * d2, mat.data(), 8 * mat.size2(), 8 * d2, d1, hipMemcpyDeviceToDevice),
-                "hipMemcpy2D");
+      dcopy2d(t.data(), 8 * d2, mat.data(), 8 * mat.size2(), 8 * d2, d1, hipMemcpyDeviceToDevice);
     mat = std::move(t);
   }
 }
@@ -82,9 +81,8 @@ inline void trim_matrices(Operators &a, const DiagInfo &diag) {   // Operators::
 inline std::vector<double> diagonal(const DMatrix &m, size_t n) {
   std::vector<double> d(n);
   if (n)
-    hip_check(hipMemcpy2D(d.data(), sizeof(double), m.data(), (m.size2() + 1) * sizeof(double), sizeof(double), n,
-                          hipMemcpyDeviceToHost),
-              "hipMemcpy2D");
+    dcopy2d(d.data(), sizeof(double), m.data(), (m.size2() + 1) * sizeof(double), sizeof(double), n,
+                          hipMemcpyDeviceToHost);
   return d;
 }
 
@@ -374,8 +372,7 @@ class NRGCalculation {
           const size_t blk = H.add_block(I, substruct.at(I));
           Sym_->hamiltonian(H, blk, I, substruct.at(I), diagprev, ops_.opch, coef_, step.N(), step.scale());
         }
-        DiagParams DP(P_.diag == "default" ? "default" : P_.diag, diagratio);
-        DP.check = false;
+        DiagParams DP(P_.diag == "default" ? "default" : P_.diag, diagratio);   // check_diag on the device
         std::vector<size_t> sizes;
         double n3 = 0;
         for (const auto &t : tasks) {

@@ -88,7 +88,7 @@ inline Curve continuous(Device &dev, const BinGrid &g, const SpectrumBins &s, do
   const auto [ne, nw] = trim(g, s.neg, discard_trim);
   auto up = [](const std::vector<double> &v) {
     DBuf<double> d(v.size() + 1);
-    if (!v.empty()) hip_check(hipMemcpy(d.get(), v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!v.empty()) dcopy(d.get(), v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice);
     return d;
   };
   DBuf<double> dpe = up(pe), dpw = up(pw), dne = up(ne), dnw = up(nw), dm = up(mesh);
@@ -98,8 +98,8 @@ inline Curve continuous(Device &dev, const BinGrid &g, const SpectrumBins &s, do
             "nrg_broaden");
   std::vector<double> vp(mesh.size()), vn(mesh.size());
   if (!mesh.empty()) {
-    hip_check(hipMemcpy(vp.data(), op.get(), sizeof(double) * mesh.size(), hipMemcpyDeviceToHost), "hipMemcpy");
-    hip_check(hipMemcpy(vn.data(), on.get(), sizeof(double) * mesh.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    dcopy(vp.data(), op.get(), sizeof(double) * mesh.size(), hipMemcpyDeviceToHost);
+    dcopy(vn.data(), on.get(), sizeof(double) * mesh.size(), hipMemcpyDeviceToHost);
   }
   Curve neg, pos;
   for (size_t k = 0; k < mesh.size(); k++) {
@@ -137,7 +137,7 @@ inline KeptPairs kept_pairs(const DiagInfo &diag, const MatrixElements &op1, con
   }
   kp.dE.alloc(E.size() + 1);
   if (!E.empty())
-    hip_check(hipMemcpy(kp.dE.get(), E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    dcopy(kp.dE.get(), E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice);
   for (const auto &[Ip, diagp] : diag)
     for (const auto &[I1, diag1] : diag) {
       const Twoinvar II{I1, Ip};
@@ -176,7 +176,7 @@ inline KeptPairs fdm_pairs(const DiagInfo &diag, const MatrixElements &op1, cons
   }
   kp.dE.alloc(E.size() + 1);
   if (!E.empty())
-    hip_check(hipMemcpy(kp.dE.get(), E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    dcopy(kp.dE.get(), E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice);
   for (const auto &[Ii, diagi] : diag)
     for (const auto &[Ij, diagj] : diag) {
       const Twoinvar II{Ij, Ii};
@@ -220,7 +220,7 @@ inline void set_rho(nrg_fdm_pair &p, const DensMatElements &rho, const Invar &Ip
 
 inline std::vector<double> to_host(const DBuf<double> &d, size_t n) {
   std::vector<double> h(n);
-  if (n) hip_check(hipMemcpy(h.data(), d.get(), sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+  if (n) dcopy(h.data(), d.get(), sizeof(double) * n, hipMemcpyDeviceToHost);
   return h;
 }
 }  // namespace detail
@@ -246,7 +246,7 @@ class FDMSpectrum {
               "nrg_fdm_bin");
     // mergeCFS of the step's chains into the running spectra
     std::vector<double> h(4 * nb);
-    hip_check(hipMemcpy(h.data(), base, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    dcopy(h.data(), base, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
     for (size_t i = 0; i < nb; i++) {
       gt_.pos[i] += h[i];
       gt_.neg[i] += h[nb + i];

@@ -101,7 +101,7 @@ class StepStore {
       for (const auto &[I, s] : e.shapes) {
         DMatrix m(s.rows, s.cols);
         if (s.rows * s.cols)
-          hip_check(hipMemcpy(m.data(), e.host.get() + s.off, 8 * s.rows * s.cols, hipMemcpyHostToDevice), "hipMemcpy");
+          dcopy(m.data(), e.host.get() + s.off, 8 * s.rows * s.cols, hipMemcpyHostToDevice);
         out.emplace(I, std::move(m));
       }
     } else if (remove) {
@@ -110,7 +110,7 @@ class StepStore {
       for (const auto &[I, m] : e.rho) {
         DMatrix c(m.size1(), m.size2());
         if (m.size1() * m.size2())
-          hip_check(hipMemcpy(c.data(), m.data(), 8 * m.size1() * m.size2(), hipMemcpyDeviceToDevice), "hipMemcpy");
+          dcopy(c.data(), m.data(), 8 * m.size1() * m.size2(), hipMemcpyDeviceToDevice);
         out.emplace(I, std::move(c));
       }
     }
@@ -176,7 +176,7 @@ class StepStore {
     for (auto &[I, eig] : e.diag) {
       const DView v = eig.vectors.dev();
       const size_t n = extent(v);
-      if (n) hip_check(hipMemcpy(e.host.get() + off, v.p, 8 * n, hipMemcpyDeviceToHost), "hipMemcpy");
+      if (n) dcopy(e.host.get() + off, v.p, 8 * n, hipMemcpyDeviceToHost);
       e.vec[I] = Shape{off, v.rows, v.cols, v.ld};
       auto &bl = e.blocks[I];
       bl.clear();
@@ -205,7 +205,7 @@ class StepStore {
     size_t off = 0;
     for (const auto &[I, m] : e.rho) {
       const size_t n = m.size1() * m.size2();
-      if (n) hip_check(hipMemcpy(e.host.get() + off, m.data(), 8 * n, hipMemcpyDeviceToHost), "hipMemcpy");
+      if (n) dcopy(e.host.get() + off, m.data(), 8 * n, hipMemcpyDeviceToHost);
       e.shapes[I] = Shape{off, m.size1(), m.size2(), m.size2()};
       off += n;
     }
@@ -219,7 +219,7 @@ class StepStore {
   static DiagInfo restore(const DiagEntry &e) {
     auto arena = std::make_shared<DBuf<double>>(e.host.size() + 1);
     if (e.host.size())
-      hip_check(hipMemcpy(arena->get(), e.host.get(), 8 * e.host.size(), hipMemcpyHostToDevice), "hipMemcpy");
+      dcopy(arena->get(), e.host.get(), 8 * e.host.size(), hipMemcpyHostToDevice);
     DiagInfo out = e.diag;
     for (auto &[I, eig] : out) {
       const Shape &s = e.vec.at(I);
