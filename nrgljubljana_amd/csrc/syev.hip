@@ -1411,114 +1411,177 @@ __global__ __launch_bounds__(256) void k_ref_E(RefArgs r, const int2 *items) {
 
 constexpr int CMAX = 64;   // largest cluster solved as one Ritz problem
 
-// clusters of eigenvalue estimates closer than delta: Ritz problem by cyclic Jacobi, rotate Xn_C
-__global__ __launch_bounds__(256) void k_ref_clusters(RefArgs r, const int *blist) {
+// One cluster: estimates [start, start + c) of block b.
+struct RefCluster {
+  int b, start, c;
+};
+
+// Clusters of consecutive eigenvalue estimates whose neighbours are closer than delta. A run longer
+// than CMAX is cut into pieces of CMAX (and a last piece); pieces of one estimate are not clusters.
+// The thread at the first index of a run walks the run and appends its pieces to `cl`.
+__global__ __launch_bounds__(256) void k_ref_find(RefArgs r, const int *blist, RefCluster *cl, int *ncl) {
   const int b = blist[blockIdx.x];
   const BlkDesc bd = r.bd[b];
   const int n = (int)bd.n;
   const double delta = ref_delta(r, b);
-  double *lt = r.lt + bd.offV;
+  const double *lt = r.lt + bd.offV;
+  for (int i = threadIdx.x; i + 1 < n; i += 256) {
+    if (i > 0 && fabs(lt[i] - lt[i - 1]) <= delta) continue;   // inside a run
+    if (!(fabs(lt[i + 1] - lt[i]) <= delta)) continue;          // a run of one
+    int end = i + 1;
+    while (end < n && fabs(lt[end] - lt[end - 1]) <= delta) end++;
+    for (int s0 = i; s0 < end; s0 += CMAX) {
+      const int c = min(CMAX, end - s0);
+      if (c >= 2) cl[atomicAdd(ncl, 1)] = RefCluster{b, s0, c};
+    }
+  }
+}
+
+// Ritz problem of one cluster C per workgroup: B = S_CC + (R_CC S_CC + S_CC R_CC) / 2, symmetrised,
+// diagonalised by parallel cyclic Jacobi in LDS (round-robin ordering: the c/2 rotations of a step act
+// on disjoint index pairs, so they are applied together, columns first, then rows); the estimates
+// l_C become its eigenvalues in ascending order and X_C <- X_C Q with Q the matching eigenvectors.
+// Workgroups past the number of clusters found return at once.
+__global__ __launch_bounds__(256) void k_ref_csolve(RefArgs r, const RefCluster *cl, const int *ncl) {
+  if ((int)blockIdx.x >= *ncl) return;
+  const RefCluster C = cl[blockIdx.x];
+  const BlkDesc bd = r.bd[C.b];
+  const int n = (int)bd.n, start = C.start, c = C.c;
   const double *S = r.S + bd.offZ, *R = r.R + bd.offZ;
   double *X = r.Xn + bd.offZ;
-  __shared__ double B[CMAX][CMAX + 1], Q[CMAX][CMAX + 1], Bt[CMAX][CMAX + 1];
+  double *lt = r.lt + bd.offV;
+  __shared__ double B[CMAX][CMAX + 1], Q[CMAX][CMAX + 1];
+  __shared__ double rc[CMAX / 2], rs[CMAX / 2], red[2][256 / WAVE];
+  __shared__ int rp[CMAX / 2], rq[CMAX / 2], perm[CMAX];
   __shared__ double ev[CMAX];
-  __shared__ int perm[CMAX];
-  __shared__ double cs[2];
-  // The scan of a cluster starting at `start` reads lt[start, end]; the cluster's new values are
-  // written to lt[start, end) only after its scan, and the next scan starts at `end`: every value
-  // read is still the estimate from before this kernel.
-  int start = 0;
-  while (start < n) {
-    int end = start + 1;
-    while (end < n && end - start < CMAX && fabs(lt[end] - lt[end - 1]) <= delta) end++;
-    const int c = end - start;
-    if (c >= 2) {
-      // B = S_CC + (R_CC S_CC + S_CC R_CC) / 2
-      for (int idx = threadIdx.x; idx < c * c; idx += 256) {
-        const int i = idx % c, j = idx / c;
-        double v = S[(size_t)(start + j) * n + start + i];
-        double rs = 0;
-        for (int k = 0; k < c; k++)
-          rs += R[(size_t)(start + k) * n + start + i] * S[(size_t)(start + j) * n + start + k] +
-                S[(size_t)(start + k) * n + start + i] * R[(size_t)(start + j) * n + start + k];
-        B[i][j] = v + 0.5 * rs;
-        Q[i][j] = i == j ? 1.0 : 0.0;
-      }
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < c * c; idx += 256) {   // symmetrise
-        const int i = idx % c, j = idx / c;
-        Bt[i][j] = 0.5 * (B[i][j] + B[j][i]);
-      }
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < c * c; idx += 256) B[idx % c][idx / c] = Bt[idx % c][idx / c];
-      __syncthreads();
-      for (int sweep = 0; sweep < 30; sweep++) {
-        double off = 0, tot = 0;
-        for (int i = 0; i < c; i++)
-          for (int j = 0; j < c; j++) {
-            tot += B[i][j] * B[i][j];
-            if (i != j) off += B[i][j] * B[i][j];
-          }
-        if (off <= 1e-34 * tot || off == 0.0) break;
-        for (int p = 0; p < c - 1; p++)
-          for (int q = p + 1; q < c; q++) {
-            if (threadIdx.x == 0) {
-              const double apq = B[p][q];
-              double cc = 1.0, ss = 0.0;
-              if (apq != 0.0) {
-                const double th = (B[q][q] - B[p][p]) / (2.0 * apq);
-                const double t = copysign(1.0, th) / (fabs(th) + sqrt(1.0 + th * th));
-                cc = 1.0 / sqrt(1.0 + t * t);
-                ss = t * cc;
-              }
-              cs[0] = cc;
-              cs[1] = ss;
-            }
-            __syncthreads();
-            const double cc = cs[0], ss = cs[1];
-            if (ss != 0.0) {
-              const int k = threadIdx.x;
-              if (k < c) {   // columns p, q
-                const double bkp = B[k][p], bkq = B[k][q];
-                B[k][p] = cc * bkp - ss * bkq;
-                B[k][q] = ss * bkp + cc * bkq;
-                const double qkp = Q[k][p], qkq = Q[k][q];
-                Q[k][p] = cc * qkp - ss * qkq;
-                Q[k][q] = ss * qkp + cc * qkq;
-              }
-              __syncthreads();
-              if (k < c) {   // rows p, q
-                const double bpk = B[p][k], bqk = B[q][k];
-                B[p][k] = cc * bpk - ss * bqk;
-                B[q][k] = ss * bpk + cc * bqk;
-              }
-            }
-            __syncthreads();
-          }
-      }
-      // ascending order of the Ritz values
-      if (threadIdx.x == 0) {
-        for (int i = 0; i < c; i++) { perm[i] = i; ev[i] = B[i][i]; }
-        for (int i = 1; i < c; i++) {
-          int j = i;
-          while (j > 0 && ev[perm[j - 1]] > ev[perm[j]]) { const int t = perm[j]; perm[j] = perm[j - 1]; perm[j - 1] = t; j--; }
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < c * c; idx += 256) {
+    const int i = idx % c, j = idx / c;
+    double rsum = 0;
+    for (int k = 0; k < c; k++)
+      rsum += R[(size_t)(start + k) * n + start + i] * S[(size_t)(start + j) * n + start + k] +
+              S[(size_t)(start + k) * n + start + i] * R[(size_t)(start + j) * n + start + k];
+    B[i][j] = S[(size_t)(start + j) * n + start + i] + 0.5 * rsum;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < c * c; idx += 256) Q[idx % c][idx / c] = 0.5 * (B[idx % c][idx / c] + B[idx / c][idx % c]);
+  __syncthreads();
+  for (int idx = tid; idx < c * c; idx += 256) {
+    const int i = idx % c, j = idx / c;
+    B[i][j] = Q[i][j];
+    Q[i][j] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int m = (c + 1) & ~1, half = m / 2;   // an odd c gets a dummy index c, paired with nothing
+  for (int sweep = 0; sweep < 30; sweep++) {
+    double off = 0, tot = 0;
+    for (int idx = tid; idx < c * c; idx += 256) {
+      const double v = B[idx % c][idx / c];
+      tot += v * v;
+      if (idx % c != idx / c) off += v * v;
+    }
+    off = wave_sum(off);
+    tot = wave_sum(tot);
+    if (tid % WAVE == 0) {
+      red[0][tid / WAVE] = off;
+      red[1][tid / WAVE] = tot;
+    }
+    __syncthreads();
+    off = 0;
+    tot = 0;
+    for (int w = 0; w < 256 / WAVE; w++) {
+      off += red[0][w];
+      tot += red[1][w];
+    }
+    if (off <= 1e-34 * tot || off == 0.0) break;   // uniform: every thread read the same sums
+    for (int step = 0; step < m - 1; step++) {
+      if (tid < half) {   // circle method: m - 1 stays, the others turn
+        int p, q;
+        if (tid == 0) {
+          p = step;
+          q = m - 1;
+        } else {
+          p = (step + tid) % (m - 1);
+          q = (step - tid + (m - 1)) % (m - 1);
         }
+        if (p > q) { const int t = p; p = q; q = t; }
+        double cc = 1.0, ss = 0.0;
+        if (q < c) {
+          const double apq = B[p][q];
+          if (apq != 0.0) {
+            const double th = (B[q][q] - B[p][p]) / (2.0 * apq);
+            const double t = copysign(1.0, th) / (fabs(th) + sqrt(1.0 + th * th));
+            cc = 1.0 / sqrt(1.0 + t * t);
+            ss = t * cc;
+          }
+        }
+        rp[tid] = p;
+        rq[tid] = q;
+        rc[tid] = cc;
+        rs[tid] = ss;
       }
       __syncthreads();
-      for (int i = threadIdx.x; i < c; i += 256) lt[start + i] = ev[perm[i]];
-      // X_C <- X_C Q(:, perm)
-      for (int row = threadIdx.x; row < n; row += 256) {
-        double xr[CMAX];
-        for (int k = 0; k < c; k++) xr[k] = X[(size_t)(start + k) * n + row];
-        for (int j = 0; j < c; j++) {
-          double v = 0;
-          for (int k = 0; k < c; k++) v += xr[k] * Q[k][perm[j]];
-          X[(size_t)(start + j) * n + row] = v;
-        }
+      for (int idx = tid; idx < half * c; idx += 256) {   // columns p, q of B and Q
+        const int k = idx / c, row = idx % c;
+        const double cc = rc[k], ss = rs[k];
+        if (ss == 0.0) continue;
+        const int p = rp[k], q = rq[k];
+        const double bp = B[row][p], bq = B[row][q];
+        B[row][p] = cc * bp - ss * bq;
+        B[row][q] = ss * bp + cc * bq;
+        const double qp = Q[row][p], qq = Q[row][q];
+        Q[row][p] = cc * qp - ss * qq;
+        Q[row][q] = ss * qp + cc * qq;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < half * c; idx += 256) {   // rows p, q of B
+        const int k = idx / c, col = idx % c;
+        const double cc = rc[k], ss = rs[k];
+        if (ss == 0.0) continue;
+        const int p = rp[k], q = rq[k];
+        const double bp = B[p][col], bq = B[q][col];
+        B[p][col] = cc * bp - ss * bq;
+        B[q][col] = ss * bp + cc * bq;
       }
       __syncthreads();
     }
-    start = end;
+  }
+  // ascending order of the Ritz values
+  if (tid == 0) {
+    for (int i = 0; i < c; i++) {
+      perm[i] = i;
+      ev[i] = B[i][i];
+    }
+    for (int i = 1; i < c; i++) {
+      int j = i;
+      while (j > 0 && ev[perm[j - 1]] > ev[perm[j]]) {
+        const int t = perm[j];
+        perm[j] = perm[j - 1];
+        perm[j - 1] = t;
+        j--;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < c; i += 256) lt[start + i] = ev[perm[i]];
+  // X_C <- X_C Q(:, perm), eigenvector k of the cluster at X[(start + k) n .. + n): 64 components at a
+  // time, staged in B (no longer needed)
+  for (int r0 = 0; r0 < n; r0 += CMAX) {
+    const int w = min(CMAX, n - r0);
+    for (int idx = tid; idx < c * CMAX; idx += 256) {
+      const int k = idx / CMAX, rr = idx % CMAX;
+      if (rr < w) B[k][rr] = X[(size_t)(start + k) * n + r0 + rr];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < c * CMAX; idx += 256) {
+      const int j = idx / CMAX, rr = idx % CMAX;
+      if (rr >= w) continue;
+      const int pj = perm[j];
+      double v = 0;
+      for (int k = 0; k < c; k++) v += B[k][rr] * Q[k][pj];
+      X[(size_t)(start + j) * n + r0 + rr] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -2299,6 +2362,8 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   const size_t olt = cv.take<double>(pl.sumn);
   const size_t osc = cv.take<double>(nblk), odscr = cv.take<double>(DS_COUNT * pl.sumn), oiscr = cv.take<int>(IS_COUNT * pl.sumn);
   const size_t omst = cv.take<MergeState>(pl.mrg_max + 1), oamax = cv.take<double>(nblk), oacc = cv.take<double>(3 * nblk);
+  const size_t ccap = (size_t)pl.sumn / 2 + 1;   // clusters have two estimates or more and do not overlap
+  const size_t ocl = cv.take<RefCluster>(ccap), oncl = cv.take<int>(1);
   if (int r = ctx->ws.ensure(cv.off)) return r;
   // FP32 workspace
   Carver cf;
@@ -2400,6 +2465,8 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   RefArgs ra;
   ra.bd = dbd; ra.A = H; ra.AX = AX; ra.S = S; ra.R = Rm; ra.E = Em;
   ra.lt = ctx->ws.as<double>(olt); ra.acc = ctx->ws.as<double>(oacc); ra.flags = aD.flags;
+  RefCluster *clist = ctx->ws.as<RefCluster>(ocl);
+  int *ncl = ctx->ws.as<int>(oncl);
   ra.X = Z0; ra.Xn = Z1;
   const GemmDesc *dR = (const GemmDesc *)(ctx->desc.as<char>(oRr));
   const int2 *rci = (const int2 *)(I + rc_off), *cci = (const int2 *)(I + col_off);
@@ -2413,6 +2480,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
     if (sweep > 0)
       NRG_TIMED(ctx, st, "k_ref_eye", hipLaunchKernelGGL(k_ref_eye, dim3((unsigned)rc_cnt), dim3(256), 0, st, ra, rci));
     NRG_HIP(hipMemsetAsync(ra.acc, 0, sizeof(double) * 3 * nblk, st));
+    NRG_HIP(hipMemsetAsync(ncl, 0, sizeof(int), st));
     for (int ph = 0; ph < 2; ph++)
       gemm_launch(ctx, st, dR, I + rg_off[dir][ph], pl.ibuf.data() + rg_off[dir][ph], rg_cnt[dir][ph]);
     NRG_TIMED(ctx, st, "k_ref_lam", hipLaunchKernelGGL(k_ref_lam, dim3((unsigned)nblk), dim3(256), 0, st, ra, I + pl.all_off));
@@ -2420,8 +2488,10 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
     NRG_TIMED(ctx, st, "k_ref_E", hipLaunchKernelGGL(k_ref_E, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
     gemm_launch(ctx, st, dR, I + rg_off[dir][3], pl.ibuf.data() + rg_off[dir][3], rg_cnt[dir][3]);
     gemm_launch(ctx, st, dR, I + rg_off[dir][2], pl.ibuf.data() + rg_off[dir][2], rg_cnt[dir][2]);
-    NRG_TIMED(ctx, st, "k_ref_clusters", hipLaunchKernelGGL(k_ref_clusters, dim3((unsigned)nblk), dim3(256), 0, st, ra,
-                                                            I + pl.all_off));
+    NRG_TIMED(ctx, st, "k_ref_find", hipLaunchKernelGGL(k_ref_find, dim3((unsigned)nblk), dim3(256), 0, st, ra,
+                                                        I + pl.all_off, clist, ncl));
+    NRG_TIMED(ctx, st, "k_ref_csolve", hipLaunchKernelGGL(k_ref_csolve, dim3((unsigned)ccap), dim3(256), 0, st, ra,
+                                                          (const RefCluster *)clist, (const int *)ncl));
   }
   NRG_HIP(hipEventRecord(ctx->ev[4], st));
   ra.X = Xbuf[nref & 1];
