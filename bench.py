@@ -1232,6 +1232,17 @@ def bench_dm(args):
     print(json.dumps(out), flush=True)
 
 
+def tdfdm_columns(d1, d2):
+    """max |difference| per column of two tdfdm files"""
+    def rd(d):
+        with open(os.path.join(d, "tdfdm")) as F:
+            lines = [l.split() for l in F if l.strip()]
+        return lines[0][1:], np.array([[float(x) for x in l] for l in lines[1:]])
+    names, a = rd(d1)
+    _, b = rd(d2)
+    return {nm: float(np.max(np.abs(a[:, i] - b[:, i]))) for i, nm in enumerate(names)}
+
+
 def bench_nrg(args):
     """A whole NRG calculation at a configuration's parameter point (workloads.NRG_CONFIGS: c1 the
     single-channel DM-NRG run, c5 the two-channel keep-4000 FDM run), through libnrgrun.so: every step
@@ -1283,6 +1294,14 @@ def bench_nrg(args):
         cmp = {}
         for f in sorted(os.listdir(runs["fp64"]["dir"])):
             if f in ("param", "data", "log", "nrg_timing.json", "DONE"):
+                continue
+            if f == "tdfdm":
+                # C_fdm = var(E) / T^2 of absolute energies (|E| ~ 4 at T = 1e-10): the rounding of the
+                # FDM weights' normalisation, ~1e-16 E^2 / T^2, is all it holds (negative in both forms, as
+                # the reference's formula gives); it is reported apart from the other columns
+                cols = tdfdm_columns(runs["mixed"]["dir"], runs["fp64"]["dir"])
+                cmp["tdfdm (T, E, F, S)"] = max(v for k, v in cols.items() if k != "C_fdm")
+                cmp["tdfdm C_fdm (rounding only)"] = cols["C_fdm"]
                 continue
             ok, err, _ = compare_files(os.path.join(runs["mixed"]["dir"], f), os.path.join(runs["fp64"]["dir"], f))
             cmp[f] = err

@@ -133,6 +133,28 @@ int nrg_check_diag_batched(nrg_ctx *ctx, int nblk, const int64_t *n, const int64
                            double eps_orth, double *dev_norm, double *dev_orth, int *info, void *stream);
 
 /*
+ * Diagonals of many square row-major device matrices in one call: replaces the element loops that
+ * set the shell density matrices, rho(i,i) = exp(-E_i/T) w_N / Z_N^D (init_rho_FDM,
+ * c++/dmnrg.hpp:145-162), and read the diagonals for their traces (check_trace_rho and the FDM
+ * trace check, c++/dmnrg.hpp:29-33 and 199-217). For each item k and i < count:
+ *   NRG_DIAG_GET  values[off + i] = A[(first + i) * (ld + 1)]
+ *   NRG_DIAG_SET  A[(first + i) * (ld + 1)] = values[off + i]
+ * `values` is host memory; a GET leaves the positions no item covers undefined. Synchronous on
+ * `stream`.
+ */
+#define NRG_DIAG_GET 0
+#define NRG_DIAG_SET 1
+typedef struct nrg_diag_item {
+  double *A;       /* device, row-major, leading dimension ld */
+  int64_t ld;
+  int64_t first;   /* first diagonal index */
+  int64_t count;   /* diagonal entries */
+  int64_t off;     /* position of the first entry in `values` */
+} nrg_diag_item;
+
+int nrg_diagonal_batched(nrg_ctx *ctx, int op, const nrg_diag_item *items, int nitems, double *values, void *stream);
+
+/*
  * One term of a grouped triple product, in the reference's row-major layout:
  *   C[m x n] += f * opA[m x k] * O[k x l] * opB[l x n]
  * with

@@ -18,7 +18,7 @@ NRG_GEMM_FP64 = 0
 NRG_GEMM_OZAKI = 1
 
 EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version", "nrg_syev_batched",
-            "nrg_check_diag_batched",
+            "nrg_check_diag_batched", "nrg_diagonal_batched",
             "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_cfs_bin", "nrg_ft_bin", "nrg_fdm_mats", "nrg_dmnrg_mats",
             "nrg_ft_mats", "nrg_temp_sum", "nrg_temp_sum_noalloc", "nrg_broaden", "nrg_hamiltonian",
             "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",

@@ -5,8 +5,11 @@
 // children, KK and DD sectors - is queued as one rotate term in a Gemm3Batch and the step is evaluated
 // in one grouped call; the density matrices stay in device memory between steps.
 #pragma once
+#include <chrono>
 #include <cstdio>
 #include <functional>
+#include <map>
+#include <string>
 
 #include "measurements.hpp"
 #include "numerics.hpp"
@@ -17,9 +20,13 @@ namespace NRG {
 // Density matrices of one step, one per subspace (c++/operators.hpp:77-83), in device memory.
 class DensMatElements : public std::map<Invar, DMatrix> {
  public:
-  double trace(const MultFnc &mult) const {
+  double trace(const MultFnc &mult) const {   // one diagonal read for all subspaces
+    std::vector<DView> v;
+    for (const auto &[I, m] : *this) v.push_back(m.view());
+    const std::vector<double> t = traces(v);
     double s = 0;
-    for (const auto &[I, m] : *this) s += mult(I) * trace_real(m);
+    size_t k = 0;
+    for (const auto &[I, m] : *this) s += mult(I) * t[k++];
     return s;
   }
 };
@@ -97,18 +104,19 @@ inline void calc_densitymatrix(DensMatElements &rho, const Store &store, const S
 // rho(i,i) = exp(-E_abs_zero(i)/T) w_N / Z_N^D on the D states [min, max), trace checked against w_N.
 inline DensMatElements init_rho_FDM(size_t N, const Store &store, const Stats &stats, const MultFnc &mult, double T) {
   DensMatElements rhoFDM;
+  std::vector<nrg_diag_item> items;
+  std::vector<double> d;
   for (const auto &[I, ds] : store[N]) {
     DMatrix m(ds.max(), ds.max());
     if (stats.ZnDNd[N] != 0.0 && ds.max() > ds.min()) {
-      std::vector<double> d(ds.max() - ds.min());
+      items.push_back(nrg_diag_item{m.data(), (int64_t)m.size2(), (int64_t)ds.min(), (int64_t)(ds.max() - ds.min()),
+                                    (int64_t)d.size()});
       for (size_t i = ds.min(); i < ds.max(); i++)
-        d[i - ds.min()] = std::exp(-ds.eig.values.abs_zero(i) / T) * stats.wn[N] / stats.ZnDNd[N];
-      const size_t ld = m.size2();
-      dcopy2d(m.data() + ds.min() * (ld + 1), (ld + 1) * sizeof(double), d.data(), sizeof(double),
-                            sizeof(double), d.size(), hipMemcpyHostToDevice);
+        d.push_back(std::exp(-ds.eig.values.abs_zero(i) / T) * stats.wn[N] / stats.ZnDNd[N]);
     }
     rhoFDM[I] = std::move(m);
   }
+  if (!items.empty()) set_diagonals(items, d);   // one transfer for all subspaces
   if (stats.wn[N] != 0.0) {
     const double tr = rhoFDM.trace(mult);
     const double diff = (tr - stats.wn[N]) / stats.wn[N];
@@ -129,28 +137,50 @@ inline DensMatElements init_rho_FDM(size_t N, const Store &store, const Stats &s
 
 // c++/dmnrg.hpp:164-197: KK contribution from rhoFDM(N) and, except at the last step, the DD
 // contribution from init_rho_FDM(N), rotated into every subspace of step N-1.
+// wall-clock seconds of the parts of a sweep, accumulated into `sub` (if given) under `key`
+class SubTimer {
+ public:
+  SubTimer(std::map<std::string, double> *sub, const char *key) : sub_(sub), key_(key), t0_(std::chrono::steady_clock::now()) {}
+  ~SubTimer() {
+    if (sub_) (*sub_)[key_] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+  }
+
+ private:
+  std::map<std::string, double> *sub_;
+  const char *key_;
+  std::chrono::steady_clock::time_point t0_;
+};
+
 inline DensMatElements calc_fulldensitymatrix_iterN(bool last_N, const DiagInfo &diag, const DensMatElements &rhoFDM,
                                                     size_t N, const Store &store, const Stats &stats,
-                                                    const Symmetry &sym, double T, Gemm3Batch &batch) {
+                                                    const Symmetry &sym, double T, Gemm3Batch &batch,
+                                                    std::map<std::string, double> *subt = nullptr) {
   DensMatElements rhoDD, rhoFDMPrev;
-  if (!last_N) rhoDD = init_rho_FDM(N, store, stats, sym.multfnc(), T);
-  for (const auto &[I, ds] : store[N - 1]) {
-    const auto subs = sym.new_subspaces(I);
-    const size_t dim = ds.kept();
-    rhoFDMPrev[I] = DMatrix(dim, dim);
-    if (!dim) continue;
-    for (size_t i = 0; i < sym.nr_combs(); i++) {
-      const Invar &sub = subs[i];
-      const double coef = sym.mult(sub) / sym.mult(I);
-      const auto y = diag.find(sub);
-      if (y == diag.end()) continue;
-      if (const auto x1 = rhoFDM.find(sub); x1 != rhoFDM.end())
-        cdmI(i, sub, x1->second, y->second, rhoFDMPrev[I], N, coef, store, batch);
-      if (!last_N)
-        if (const auto x2 = rhoDD.find(sub); x2 != rhoDD.end())
-          cdmI(i, sub, x2->second, y->second, rhoFDMPrev[I], N, coef, store, batch);
+  if (!last_N) {
+    SubTimer t(subt, "FDM.init_rho");
+    rhoDD = init_rho_FDM(N, store, stats, sym.multfnc(), T);
+  }
+  {
+    SubTimer tq(subt, "FDM.queue");
+    for (const auto &[I, ds] : store[N - 1]) {
+      const auto subs = sym.new_subspaces(I);
+      const size_t dim = ds.kept();
+      rhoFDMPrev[I] = DMatrix(dim, dim);
+      if (!dim) continue;
+      for (size_t i = 0; i < sym.nr_combs(); i++) {
+        const Invar &sub = subs[i];
+        const double coef = sym.mult(sub) / sym.mult(I);
+        const auto y = diag.find(sub);
+        if (y == diag.end()) continue;
+        if (const auto x1 = rhoFDM.find(sub); x1 != rhoFDM.end())
+          cdmI(i, sub, x1->second, y->second, rhoFDMPrev[I], N, coef, store, batch);
+        if (!last_N)
+          if (const auto x2 = rhoDD.find(sub); x2 != rhoDD.end())
+            cdmI(i, sub, x2->second, y->second, rhoFDMPrev[I], N, coef, store, batch);
+      }
     }
   }
+  SubTimer tg(subt, "FDM.gemm");
   batch.flush();
   return rhoFDMPrev;
 }
@@ -161,11 +191,17 @@ inline DensMatElements calc_fulldensitymatrix_iterN(bool last_N, const DiagInfo 
 inline void calc_fulldensitymatrix(DensMatElements &rhoFDM, const Store &store, const Stats &stats, const Symmetry &sym,
                                    const std::vector<const DiagInfo *> &diags, size_t Ninit, size_t Nmax, double T,
                                    Device &dev,
-                                   const std::function<void(size_t, const DensMatElements &)> &keep = nullptr) {
+                                   const std::function<void(size_t, const DensMatElements &)> &keep = nullptr,
+                                   std::map<std::string, double> *subt = nullptr) {
   Gemm3Batch batch(dev);
   for (size_t N = Nmax - 1; N > Ninit; N--) {
-    auto rhoFDMPrev = calc_fulldensitymatrix_iterN(N == Nmax - 1, *diags[N], rhoFDM, N, store, stats, sym, T, batch);
-    const double tr = rhoFDMPrev.trace(sym.multfnc());
+    auto rhoFDMPrev =
+        calc_fulldensitymatrix_iterN(N == Nmax - 1, *diags[N], rhoFDM, N, store, stats, sym, T, batch, subt);
+    double tr;
+    {
+      SubTimer tt(subt, "FDM.trace");
+      tr = rhoFDMPrev.trace(sym.multfnc());
+    }
     double expected = 0.0;
     for (size_t n = N; n < Nmax; n++) expected += stats.wn[n];
     const double diff = (tr - expected) / expected;
@@ -175,7 +211,11 @@ inline void calc_fulldensitymatrix(DensMatElements &rhoFDM, const Store &store, 
                     N, tr, expected);
       throw std::runtime_error(msg);
     }
-    if (keep) keep(N - 1, rhoFDMPrev);
+    if (keep) {
+      SubTimer tk(subt, "FDM.keep");
+      keep(N - 1, rhoFDMPrev);
+    }
+    SubTimer tf(subt, "FDM.free");
     rhoFDM.swap(rhoFDMPrev);
   }
 }

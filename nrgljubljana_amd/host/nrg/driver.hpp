@@ -79,10 +79,9 @@ inline void trim_matrices(Operators &a, const DiagInfo &diag) {   // Operators::
 }
 // the diagonal of a device matrix
 inline std::vector<double> diagonal(const DMatrix &m, size_t n) {
-  std::vector<double> d(n);
-  if (n)
-    dcopy2d(d.data(), sizeof(double), m.data(), (m.size2() + 1) * sizeof(double), sizeof(double), n,
-                          hipMemcpyDeviceToHost);
+  std::vector<double> d(n + 1);
+  if (n) get_diagonals({nrg_diag_item{m.data(), (int64_t)m.size2(), 0, (int64_t)n, 0}}, d);
+  d.resize(n);
   return d;
 }
 
@@ -892,8 +891,11 @@ class NRGCalculation {
     Step step(P_, RUNTYPE::NRG);
     step.set_last();
     stats_.fdm.GS_energy = stats_.GS_energy;
-    calc_ZnD(*store_, stats_.fdm, *Sym_, P_.T);
-    fdm_thermodynamics(*store_, stats_.fdm, *Sym_, P_.T);
+    {
+      SubTimer ts(&timing_.s, "FDM.stats");
+      calc_ZnD(*store_, stats_.fdm, *Sym_, P_.T);
+      fdm_thermodynamics(*store_, stats_.fdm, *Sym_, P_.T);
+    }
     TDFile tdfdm(dir_ + "/tdfdm", {"T", "E_fdm", "C_fdm", "F_fdm", "S_fdm"}, P_.prec_td, P_.width_td);
     tdfdm.set("T", P_.T);
     tdfdm.set("F_fdm", stats_.fdm.F_fdm);
@@ -905,13 +907,18 @@ class NRGCalculation {
     DensMatElements keep;
     for (const auto &[I, m] : rhoFDM) keep.emplace(I, clone(m));
     steps_.save_rho(step.lastndx(), "rhofdm", std::move(keep));
-    const auto d = loaded_steps();
+    std::vector<DiagInfo> d;
+    {
+      SubTimer tl(&timing_.s, "FDM.load");
+      d = loaded_steps();
+    }
     calc_fulldensitymatrix(rhoFDM, *store_, stats_.fdm, *Sym_, pointers(d), P_.Ninit, P_.Nmax, P_.T, dev_,
                            [this](size_t N, const DensMatElements &r) {
                              DensMatElements c;
                              for (const auto &[I, m] : r) c.emplace(I, clone(m));
                              steps_.save_rho(N, "rhofdm", std::move(c));
-                           });
+                           },
+                           &timing_.s);
     timing_.s["FDM"] += seconds_since(t0);
   }
 

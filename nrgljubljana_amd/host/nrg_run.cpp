@@ -55,16 +55,21 @@ extern "C" int nrg_run_dir(const char *dir, const char *tables_dir, int device, 
       if (!logf) throw std::runtime_error(std::string("cannot open log file ") + log_path);
     }
     std::ostream &log = log_path ? (std::ostream &)logf : std::cout;
-    NRG::Device dev(device);
-    NRG::NRGCalculation calc(dir, tables, dev, log);
-    calc.run();
-    write_timing(dir, calc.timing());
+    {
+      NRG::Device dev(device);
+      NRG::NRGCalculation calc(dir, tables, dev, log);
+      calc.run();
+      write_timing(dir, calc.timing());
+    }
+    NRG::DevicePool::get().trim();   // every device buffer of the run has been released
     return 0;
   } catch (const std::exception &e) {
     last_error = e.what();
+    NRG::DevicePool::get().trim();
     return 1;
   } catch (...) {
     last_error = "unknown exception";
+    NRG::DevicePool::get().trim();
     return 1;
   }
 }
