@@ -96,12 +96,37 @@ def measured_traffic(kernel, args):
     return k["bytes"] if k else None
 
 
+def host_cores():
+    """CPU threads this job may use: the affinity mask, bounded by OMP_NUM_THREADS when the host sets it
+    (a shared host grants each GPU job a share of its cores that way)."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
+    return max(1, cores)
+
+
+def host_info():
+    """nproc, the affinity mask's size, OMP_NUM_THREADS and the CPU model (lscpu's 'Model name')."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as F:
+            for line in F:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
 def cpu_baseline(sizes, off_H, arena, off_w, flops, budget_s=30.0):
     """The oracle (C restatement: OpenMP over blocks, largest first, LAPACK dsyev per block) timed on
     this host's cores in the two thread layouts of the reference; the better one is reported."""
     import oracle
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
+    cores = host_cores()
     res = {}
     for label, threads, blas in (("diagth=ncores,blas=1", cores, 1), ("diagth=1,blas=ncores", 1, cores)):
         a = arena.copy()
@@ -113,10 +138,10 @@ def cpu_baseline(sizes, off_H, arena, off_w, flops, budget_s=30.0):
         if dt > budget_s:
             break
     best = min(res, key=res.get)
-    return {"value": flops / res[best] / 1e9, "unit": "GFLOP/s", "cores": cores if "ncores,blas=1" in best or
-            "blas=ncores" in best else 1, "kind": "port",
+    return {"value": flops / res[best] / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
             "sample": f"the full 256-block batch once, oracle dsyev per block ({best}); seconds: "
-                      + ", ".join(f"{k}={v:.3f}" for k, v in res.items())}
+                      + ", ".join(f"{k}={v:.3f}" for k, v in res.items()),
+            "host": host_info()}
 
 
 def bench_eig(args):
@@ -197,7 +222,16 @@ def bench_eig(args):
     for k, (tms, cnt) in sorted(kt.items(), key=lambda x: -x[1][0]):
         log(f"  {k:14s} {tms / 2:9.3f} ms/step  {cnt // 2:5d} launches/step")
     dom = max(kt.items(), key=lambda x: x[1][0])[0] if kt else None
-    roofline = None
+    # the call against the FP64 peak: algorithmic flops (10/3 n^3 per block) over the call's wall time
+    # (HIP events at its first and last launch, on the library's stream)
+    ach_tf = flops / (syev_ms * 1e-3) / 1e12 if syev_ms > 0 else None
+    roofline = {"bound": "mfma", "kernel": "nrg_syev_batched (whole call)",
+                "achieved": round(ach_tf, 4) if ach_tf else None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5) if ach_tf else None, "traffic": None,
+                "call_ms": round(syev_ms, 3), "flops_per_call": flops,
+                "note": "algorithmic flops / call wall time / FP64 peak; the per-launch HBM model of the dominant "
+                        "kernel is roofline_dominant_kernel"}
+    roofdom = None
     rb = reduction_bytes(sizes, groups=red_groups())
     if dom in rb:
         tot_bytes, nl = rb[dom]
@@ -206,18 +240,12 @@ def bench_eig(args):
             log(f"warning: {dom} launched {cnt // 2} times per step, schedule model says {nl}")
         avg_s = tms / cnt * 1e-3
         ach = (tot_bytes / nl) / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
-                    "bytes_per_launch": round(tot_bytes / nl), "launches_per_step": nl,
-                    "avg_launch_us": round(avg_s * 1e6, 3)}
-        if dom == "k_col_prep":
-            roofline["note"] = ("k_col_prep moves few bytes per launch: its time is the chain of dependent memory "
-                                "round trips and block reductions of one reduced column (one workgroup per block), "
-                                "so its HBM fraction is low by construction; the bandwidth-bound kernel of the "
-                                "reduction is k_col_symv (roofline_other)")
-    elif dom is not None:
-        roofline = {"bound": "mfma", "kernel": dom, "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": None, "traffic": None}
+        roofdom = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
+                   "bytes_per_launch": round(tot_bytes / nl), "launches_per_step": nl,
+                   "avg_launch_us_events": round(avg_s * 1e6, 3),
+                   "note": "average launch time from HIP events, which add about 2 us per launch over the kernel "
+                           "trace's durations (an upper bound on time, so a lower bound on the fraction)"}
     # the other reduction kernel, same accounting
     others = {}
     for k, (tot_bytes, nl) in rb.items():
@@ -255,6 +283,7 @@ def bench_eig(args):
         "max_orth_err": orth_err,
         "kernels_ms_per_step": {k: round(v[0] / 2, 3) for k, v in kt.items()},
         "roofline": roofline,
+        "roofline_dominant_kernel": roofdom,
         "roofline_other": others,
         "cpu_baseline": None,
     }
@@ -363,7 +392,7 @@ def bench_recalc(args):
     if not args.no_cpu:
         # oracle dgemm on the host cores over a bounded sample of the table
         L = oracle.lib()
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores = host_cores()
         L.oracle_blas_threads(cores)
         sample = table
         sf = flops
@@ -1053,7 +1082,7 @@ def bench_step(args):
     if not args.no_cpu:
         # the oracle chain on the host: assembly (1 core), dsyev per block (OpenMP over blocks), recalc
         # dgemm pairs (threaded BLAS), FDM loops (1 core)
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores = host_cores()
         hk = {}
 
         def hptr(a):
@@ -1199,7 +1228,7 @@ def bench_dm(args):
     cpu = None
     if not args.no_cpu:
         L = oracle.lib()
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores = host_cores()
         L.oracle_blas_threads(cores)
         hterms, hold, hi = [], [], 0
         for sub in step:
@@ -1335,7 +1364,7 @@ def nrg_cpu_sample(blocks, budget_s=20.0):
     block orders of the run's most expensive step, largest first, until the budget is spent."""
     import oracle
     step = max(blocks, key=lambda b: sum(float(n) ** 3 for n in b))
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = host_cores()
     rng = np.random.default_rng(5)
     done, flops, t_tot = 0, 0.0, 0.0
     for n in sorted(step, reverse=True):
@@ -1349,7 +1378,8 @@ def nrg_cpu_sample(blocks, budget_s=20.0):
         done += 1
     return {"value": round(flops / t_tot / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
             "sample": f"{done} of the {len(step)} blocks of the run's most expensive step (largest first, orders "
-                      f"{max(step)} down), oracle dsyev with {cores} BLAS threads, {t_tot:.1f} s"}
+                      f"{max(step)} down), oracle dsyev with {cores} BLAS threads, {t_tot:.1f} s",
+            "host": host_info()}
 
 
 def main():

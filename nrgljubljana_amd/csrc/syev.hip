@@ -1284,20 +1284,21 @@ __global__ void k_convert(const double *src, T *dst, size_t count) {
 // ------------------------------------------------------------------------------------------
 // 6. Mixed form: Ogita-Aishima refinement in FP64 of the FP32 eigenvectors.
 //    For X with A X ~ X L:  R = I - X^T X,  S = X^T A X,  l_i = s_ii / (1 - r_ii),
-//    E_ij = (s_ij + l_j r_ij) / (l_j - l_i) if |l_i - l_j| > delta, else r_ij / 2,
+//    E_ij = (s_ij + l_j r_ij) / (l_j - l_i) for pairs of different clusters, else r_ij / 2,
 //    X <- X (I + E + K^2 / 2) with K the skew part of E (the second-order term of exp(K), which
-//    keeps the orthogonality error cubic instead of quadratic in K);
-//    delta = 2 (||S - diag(l)|| + ||A|| ||R||) (Frobenius bounds).
-//    Eigenvalues closer than delta form clusters whose Ritz problem
-//    B = S_CC + (R_CC S_CC + S_CC R_CC) / 2 is solved by Jacobi and rotates X_C.
+//    keeps the orthogonality error cubic instead of quadratic in K).
+//    The close pairs are those of one cluster: a run of neighbouring estimates linked by their own
+//    coupling (ref_linked). A cluster's Ritz problem B = S_CC + (R_CC S_CC + S_CC R_CC) / 2 is solved
+//    by Jacobi and rotates X_C.
 // ------------------------------------------------------------------------------------------
 struct RefArgs {
   const BlkDesc *bd;
   const double *A;      // scaled symmetric input, full storage (the H arena)
   double *X, *Xn, *AX, *S, *R, *E;   // n x n per block at offZ
   double *lt;           // eigenvalue estimates, n per block at offV
-  double *acc;          // per block: ||S - diag l||_F^2, ||R||_F^2, max |l|
+  double *acc;          // per block: [2] max |l| ([0], [1] unused)
   int *flags;
+  int *rid;             // per estimate (n per block at offV): first index of its cluster run
 };
 
 // X = (double) V32, R = I
@@ -1340,37 +1341,23 @@ __global__ __launch_bounds__(256) void k_ref_lam(RefArgs r, const int *blist) {
   if (threadIdx.x == 0) r.acc[3 * b + 2] = mx;
 }
 
-// Frobenius norms ||S - diag l||^2 and ||R||^2, 16 columns per item
-__global__ __launch_bounds__(256) void k_ref_norms(RefArgs r, const int2 *items) {
-  const int2 it = items[blockIdx.x];
-  const BlkDesc bd = r.bd[it.x];
-  const int n = (int)bd.n;
-  const int c0 = it.y * 16, c1 = min(n, c0 + 16);
-  __shared__ double red[4];
-  double s2 = 0, r2 = 0;
-  for (int c = c0; c < c1; c++)
-    for (int i = threadIdx.x; i < n; i += 256) {
-      const size_t k = bd.offZ + (size_t)c * n + i;
-      const double sv = r.S[k] - (i == c ? r.lt[bd.offV + i] : 0.0);
-      s2 += sv * sv;
-      r2 += r.R[k] * r.R[k];
-    }
-  s2 = block_sum<256>(s2, red);
-  r2 = block_sum<256>(r2, red);
-  if (threadIdx.x == 0) {
-    atomicAdd(&r.acc[3 * it.x], s2);
-    atomicAdd(&r.acc[3 * it.x + 1], r2);
-  }
-}
-
-// Cluster threshold. Besides the Ogita-Aishima bound, pairs closer than 1e-6 ||A|| are always
-// treated as a cluster: there the correction (s_ij + l_j r_ij) / (l_j - l_i) would be dominated by
-// the rounding noise of s_ij (~ sqrt(n) eps ||A||), and its skew-symmetric part costs
-// orthogonality quadratically; the cluster's Ritz step is exactly orthogonal instead.
+// Which estimates are clustered. Ogita-Aishima's global threshold 2 (||S - diag l||_2 + ||A|| ||R||_2),
+// bounded by Frobenius norms, overstates the coupling by up to sqrt(n) (about 75 at n = 5500); on NRG
+// blocks (dense spectra, half the levels in exactly degenerate pairs) it merged a thousand estimates
+// into one run, and no global value served both those blocks and the synthetic clustered spectra. So
+// neighbours i-1, i are linked by their own coupling: when their gap is at most
+// max(CLUSTER_FLOOR ||A||, CLUSTER_KAPPA (|s| + |l| |r|)) of the pair (symmetric parts). Linked runs
+// are the clusters; every other pair takes the first-order correction, which is then below
+// 1 / CLUSTER_KAPPA for neighbours.
 constexpr double CLUSTER_FLOOR = 1e-6;
-__device__ __forceinline__ double ref_delta(const RefArgs &r, int b) {
-  const double anorm = r.acc[3 * b + 2];
-  return fmax(2.0 * (sqrt(r.acc[3 * b]) + anorm * sqrt(r.acc[3 * b + 1])), CLUSTER_FLOOR * anorm);
+constexpr double CLUSTER_KAPPA = 8.0;
+__device__ __forceinline__ bool ref_linked(const RefArgs &r, const BlkDesc &bd, int n, int i, double anorm) {
+  const double *lt = r.lt + bd.offV;
+  const double gap = fabs(lt[i] - lt[i - 1]);
+  const size_t k = bd.offZ + (size_t)i * n + (i - 1), kt = bd.offZ + (size_t)(i - 1) * n + i;
+  const double sij = 0.5 * (r.S[k] + r.S[kt]), rij = 0.5 * (r.R[k] + r.R[kt]);
+  const double coup = fabs(sij) + fabs(lt[i]) * fabs(rij);
+  return gap <= fmax(CLUSTER_FLOOR * anorm, CLUSTER_KAPPA * coup);
 }
 
 // E' = I + E, 16 columns per item
@@ -1379,8 +1366,8 @@ __global__ __launch_bounds__(256) void k_ref_E(RefArgs r, const int2 *items) {
   const BlkDesc bd = r.bd[it.x];
   const int n = (int)bd.n;
   const int c0 = it.y * 16, c1 = min(n, c0 + 16);
-  const double delta = ref_delta(r, it.x);
   const double *lt = r.lt + bd.offV;
+  const int *rid = r.rid + bd.offV;
   // S and R enter through their symmetric parts: the computed X^T (A X) and X^T X are symmetric
   // only up to rounding, and the antisymmetric rest, divided by l_j - l_i, would leave
   // E + E^T != R and stall the orthogonality at (s_ij - s_ji) / gap.
@@ -1394,7 +1381,7 @@ __global__ __launch_bounds__(256) void k_ref_E(RefArgs r, const int2 *items) {
         e = 1.0 + 0.5 * rij;
       } else {
         const double li = lt[i];
-        if (fabs(li - lj) > delta) {
+        if (rid[i] != rid[j]) {
           const double sij = 0.5 * (r.S[k] + r.S[kt]);
           const double eij = (sij + lj * rij) / (lj - li), eji = (sij + li * rij) / (li - lj);
           e = eij;
@@ -1416,23 +1403,28 @@ struct RefCluster {
   int b, start, c;
 };
 
-// Clusters of consecutive eigenvalue estimates whose neighbours are closer than delta. A run longer
-// than CMAX is cut into pieces of CMAX (and a last piece); pieces of one estimate are not clusters.
-// The thread at the first index of a run walks the run and appends its pieces to `cl`.
-__global__ __launch_bounds__(256) void k_ref_find(RefArgs r, const int *blist, RefCluster *cl, int *ncl) {
+// Clusters: runs of consecutive estimates linked by ref_linked. A run longer than CMAX is cut into
+// pieces of CMAX (and a last piece); pieces of one estimate are not clusters. With `half` set the first
+// piece of a run has CMAX / 2 estimates, so that consecutive sweeps cut long runs at different places.
+// rid receives, for every estimate, the first index of its run (its own index when unlinked): k_ref_E
+// treats two estimates of one run as a cluster pair. The thread at the first index of a run walks it.
+__global__ __launch_bounds__(256) void k_ref_find(RefArgs r, const int *blist, RefCluster *cl, int *ncl, int half) {
   const int b = blist[blockIdx.x];
   const BlkDesc bd = r.bd[b];
   const int n = (int)bd.n;
-  const double delta = ref_delta(r, b);
-  const double *lt = r.lt + bd.offV;
-  for (int i = threadIdx.x; i + 1 < n; i += 256) {
-    if (i > 0 && fabs(lt[i] - lt[i - 1]) <= delta) continue;   // inside a run
-    if (!(fabs(lt[i + 1] - lt[i]) <= delta)) continue;          // a run of one
+  const double anorm = r.acc[3 * b + 2];
+  int *rid = r.rid + bd.offV;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (i > 0 && ref_linked(r, bd, n, i, anorm)) continue;   // inside a run
+    rid[i] = i;
+    if (i + 1 >= n || !ref_linked(r, bd, n, i + 1, anorm)) continue;   // a run of one
     int end = i + 1;
-    while (end < n && fabs(lt[end] - lt[end - 1]) <= delta) end++;
-    for (int s0 = i; s0 < end; s0 += CMAX) {
-      const int c = min(CMAX, end - s0);
+    while (end < n && ref_linked(r, bd, n, end, anorm)) rid[end++] = i;
+    for (int s0 = i; s0 < end;) {
+      const int len = (half && s0 == i && end - i > CMAX) ? CMAX / 2 : CMAX;
+      const int c = min(len, end - s0);
       if (c >= 2) cl[atomicAdd(ncl, 1)] = RefCluster{b, s0, c};
+      s0 += c;
     }
   }
 }
@@ -2331,7 +2323,11 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
 // Mixed form: FP32 reduction and back-transformation on the FP32 matrix cores, FP64 divide and
 // conquer on the tridiagonal matrix, then NREF FP64 refinement sweeps.
 // ------------------------------------------------------------------------------------------
-constexpr int NREF_DEFAULT = 3;   // two sweeps reach FP64 orthogonality but leave ~1e-11 eigenvalue errors
+// Sweeps of the mixed form. Two leave ~1e-11 eigenvalue errors on clustered spectra; on two-channel
+// NRG blocks of order ~5500 (half the spectrum in exactly degenerate pairs, a run of 1145 estimates
+// closer than the first sweep's delta) three leave eigenvalue errors of 1e-9 and normalisation errors of
+// 3e-11, four reach 1e-13 and 1e-14.
+constexpr int NREF_DEFAULT = 4;
 
 // number of refinement sweeps: NRG_MIXED_SWEEPS in the environment (1..6), default NREF_DEFAULT
 static int refine_sweeps() {
@@ -2364,6 +2360,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   const size_t omst = cv.take<MergeState>(pl.mrg_max + 1), oamax = cv.take<double>(nblk), oacc = cv.take<double>(3 * nblk);
   const size_t ccap = (size_t)pl.sumn / 2 + 1;   // clusters have two estimates or more and do not overlap
   const size_t ocl = cv.take<RefCluster>(ccap), oncl = cv.take<int>(1);
+  const size_t orid = cv.take<int>(pl.sumn);
   if (int r = ctx->ws.ensure(cv.off)) return r;
   // FP32 workspace
   Carver cf;
@@ -2467,6 +2464,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   ra.lt = ctx->ws.as<double>(olt); ra.acc = ctx->ws.as<double>(oacc); ra.flags = aD.flags;
   RefCluster *clist = ctx->ws.as<RefCluster>(ocl);
   int *ncl = ctx->ws.as<int>(oncl);
+  ra.rid = ctx->ws.as<int>(orid);
   ra.X = Z0; ra.Xn = Z1;
   const GemmDesc *dR = (const GemmDesc *)(ctx->desc.as<char>(oRr));
   const int2 *rci = (const int2 *)(I + rc_off), *cci = (const int2 *)(I + col_off);
@@ -2484,12 +2482,11 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
     for (int ph = 0; ph < 2; ph++)
       gemm_launch(ctx, st, dR, I + rg_off[dir][ph], pl.ibuf.data() + rg_off[dir][ph], rg_cnt[dir][ph]);
     NRG_TIMED(ctx, st, "k_ref_lam", hipLaunchKernelGGL(k_ref_lam, dim3((unsigned)nblk), dim3(256), 0, st, ra, I + pl.all_off));
-    NRG_TIMED(ctx, st, "k_ref_norms", hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
+    NRG_TIMED(ctx, st, "k_ref_find", hipLaunchKernelGGL(k_ref_find, dim3((unsigned)nblk), dim3(256), 0, st, ra,
+                                                        I + pl.all_off, clist, ncl, sweep & 1));
     NRG_TIMED(ctx, st, "k_ref_E", hipLaunchKernelGGL(k_ref_E, dim3((unsigned)col_cnt), dim3(256), 0, st, ra, cci));
     gemm_launch(ctx, st, dR, I + rg_off[dir][3], pl.ibuf.data() + rg_off[dir][3], rg_cnt[dir][3]);
     gemm_launch(ctx, st, dR, I + rg_off[dir][2], pl.ibuf.data() + rg_off[dir][2], rg_cnt[dir][2]);
-    NRG_TIMED(ctx, st, "k_ref_find", hipLaunchKernelGGL(k_ref_find, dim3((unsigned)nblk), dim3(256), 0, st, ra,
-                                                        I + pl.all_off, clist, ncl));
     NRG_TIMED(ctx, st, "k_ref_csolve", hipLaunchKernelGGL(k_ref_csolve, dim3((unsigned)ccap), dim3(256), 0, st, ra,
                                                           (const RefCluster *)clist, (const int *)ncl));
   }

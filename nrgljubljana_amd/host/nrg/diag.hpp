@@ -6,6 +6,8 @@
 // is the batching point one level up (c++/core.hpp:90-120): every block of a step in one
 // nrg_syev_batched call, the eigenvectors left in device memory for recalc and the density matrices.
 #pragma once
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "eigen.hpp"
@@ -81,7 +83,7 @@ struct BatchResult {
 // |<v_r,v_s>| < 1e-10
 inline void check_diag_device(Device &dev, const std::vector<int64_t> &n, const std::vector<int64_t> &rows,
                               const std::vector<int64_t> &off, const double *V, const std::vector<int64_t> &offw,
-                              const double *w) {
+                              const double *w, size_t *first_bad = nullptr) {
   const size_t nb = n.size();
   std::vector<double> dn(nb), dof(nb);
   std::vector<int> info(nb, 0);
@@ -90,10 +92,13 @@ inline void check_diag_device(Device &dev, const std::vector<int64_t> &n, const 
   if (r == NRG_OK) return;
   if (r != NRG_E_NOCONV) nrg_check(r, "nrg_check_diag_batched");
   for (size_t b = 0; b < nb; b++)
-    if (info[b])
-      throw std::runtime_error("check_diag failed for block " + std::to_string(b) + " (dim " + std::to_string(n[b]) +
-                               "): max |<v,v> - 1| = " + std::to_string(dn[b]) + ", max |<v_r,v_s>| = " +
-                               std::to_string(dof[b]) + (info[b] == 3 ? ", non-finite eigenvalue" : ""));
+    if (info[b]) {
+      if (first_bad) *first_bad = b;
+      char msg[200];
+      std::snprintf(msg, sizeof msg, "check_diag failed for block %zu (dim %lld): max |<v,v> - 1| = %.3e, max |<v_r,v_s>| = %.3e%s",
+                    b, (long long)n[b], dn[b], dof[b], info[b] == 3 ? ", non-finite eigenvalue" : "");
+      throw std::runtime_error(msg);
+    }
 }
 
 // nrg_syev_batched on blocks already in a device arena (block b: n[b] x n[b] at off[b]); with
@@ -113,13 +118,38 @@ inline BatchResult syev_arena(Device &dev, std::shared_ptr<DBuf<double>> arena, 
   }
   DBuf<double> w((size_t)totw + 1);
   std::vector<int> info(nb, 0);
+  // diagnostic: with NRG_DUMP_DIR set, the input blocks are kept on the host, and a block that fails
+  // check_diag is written to NRG_DUMP_DIR/check_diag_block_<b>.bin (int64 order, then the row-major
+  // matrix as handed to the eigensolver)
+  const char *dump_dir = check_rows ? std::getenv("NRG_DUMP_DIR") : nullptr;
+  std::vector<double> input;
+  if (dump_dir) {
+    const int64_t tot = nb ? r.off[nb - 1] + r.n[nb - 1] * r.n[nb - 1] : 0;
+    input.resize((size_t)tot);
+    if (tot) dcopy(input.data(), r.arena->get(), sizeof(double) * (size_t)tot, hipMemcpyDeviceToHost);
+  }
   nrg_check(nrg_syev_batched(dev.ctx(), (int)nb, r.n.data(), r.off.data(), r.arena->get(), offw.data(), w.get(), mode,
                              info.data(), dev.stream()),
             "nrg_syev_batched");
   for (size_t b = 0; b < nb; b++)
     if (info[b] != 0)   // the reference throws on INFO != 0 (c++/diag.hpp:61-82)
       throw std::runtime_error("diagonalisation of block " + std::to_string(b) + " failed, info=" + std::to_string(info[b]));
-  if (check_rows) check_diag_device(dev, r.n, *check_rows, r.off, r.arena->get(), offw, w.get());
+  if (check_rows) {
+    size_t bad = nb;
+    try {
+      check_diag_device(dev, r.n, *check_rows, r.off, r.arena->get(), offw, w.get(), &bad);
+    } catch (const std::runtime_error &) {
+      if (dump_dir && bad < nb) {
+        const std::string fn = std::string(dump_dir) + "/check_diag_block_" + std::to_string(bad) + ".bin";
+        if (FILE *f = std::fopen(fn.c_str(), "wb")) {
+          std::fwrite(&r.n[bad], sizeof(int64_t), 1, f);
+          std::fwrite(input.data() + r.off[bad], sizeof(double), (size_t)(r.n[bad] * r.n[bad]), f);
+          std::fclose(f);
+        }
+      }
+      throw;
+    }
+  }
   std::vector<double> wh((size_t)totw);
   dcopy(wh.data(), w.get(), sizeof(double) * (size_t)totw, hipMemcpyDeviceToHost);
   r.val.resize(nb);
