@@ -125,7 +125,7 @@ struct nrg_ctx {
   nrg::KTimer kt;
   // extra streams for independent work of one call (default priority, created on first use),
   // forked from and joined back into the caller's stream with the events
-  static constexpr int NAUX = 3;
+  static constexpr int NAUX = 4;   // the last one carries the eigensolver's two-stage chain
   hipStream_t aux[NAUX] = {};
   hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {};
   // the eigensolver's host-side work lists, kept between calls so that their memory is reused

@@ -49,7 +49,10 @@ inline int panel_in_slot(int q, int P) { return q < P ? q : -1; }
 
 struct BlkDesc {
   int64_t n, offA, offZ, offV, offWp, offH, offw, offYp, offDt;
+  int64_t offAB, offV2, offT2;   // two-stage reduction: band, bulge-chasing reflectors and scalars
   int L, P;   // P: panels reduced by the column kernels (before the tail)
+  int two;    // 1: reduced in two stages (dense -> band -> tridiagonal)
+  int roff;   // reflector j of the reduction starts at row j + roff (1, or SB in two stages)
 };
 
 template <typename T>
@@ -71,6 +74,11 @@ struct SyevArgs {
   int *flags;
   double *amax;    // per-block max-norm of the input (bit pattern updated with atomicMax)
   double *H, *w;
+  // two-stage reduction (syev_sbr.hpp)
+  T *sbV, *sbY, *sbW;   // explicit panel reflectors V, Y = A22 V and W, n x SB per block (ld n)
+  T *sbT;               // T factor of each block's current panel, SB x SB
+  T *sbAB;              // band storage, 2 SB x n per block
+  T *sbV2, *sbT2;       // bulge-chasing reflectors and their scalars
 };
 
 
@@ -476,6 +484,8 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
   __syncthreads();
   if (tid < 64 && rin) yp[(size_t)J * n + r] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
+
+#include "syev_sbr.hpp"
 
 // ------------------------------------------------------------------------------------------
 // 2c. tail of the reduction: the trailing matrix of order m <= TAIL left after the last panel
@@ -1149,18 +1159,18 @@ __global__ __launch_bounds__(256) void k_dc_fill(SyevArgs<double> a, const int2 
 // ------------------------------------------------------------------------------------------
 constexpr int KBT = 128;
 
-// Y (column j = reflector j: zeros down to row j, 1 at row j+1, v below), 64 columns per item
+// Y (column j = reflector j: zeros above row j + roff, 1 there, v below), 64 columns per item
 template <typename T>
 __global__ __launch_bounds__(256) void k_bt_ymat(SyevArgs<T> a, const int2 *items) {
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
-  const int n = (int)bd.n;
-  const int j0 = it.y * 64, j1 = min(n - 1, j0 + 64);
+  const int n = (int)bd.n, ro = bd.roff;
+  const int j0 = it.y * 64, j1 = min(n - ro, j0 + 64);
   const T *A = a.A + bd.offA;
   T *Y = a.Ym + bd.offZ;
   for (int j = j0; j < j1; j++)
     for (int r = threadIdx.x; r < n; r += 256)
-      Y[(size_t)j * n + r] = r == j + 1 ? (T)1 : (r > j + 1 ? A[(size_t)j * n + r] : (T)0);
+      Y[(size_t)j * n + r] = r == j + ro ? (T)1 : (r > j + ro ? A[(size_t)j * n + r] : (T)0);
 }
 
 // T_s (kb x kb, upper triangular, column-major) of super-panel s from G_s = Y_s^T Y_s and the
@@ -1173,7 +1183,7 @@ __global__ __launch_bounds__(256) void k_bt_tbig(SyevArgs<T> a, const int2 *item
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
   const int n = (int)bd.n, sp = it.y;
-  const int kb = min(KBT, (n - 1) - sp * KBT);
+  const int kb = min(KBT, (n - bd.roff) - sp * KBT);
   const size_t off = (size_t)toff[it.x] + (size_t)sp * KBT * KBT;
   const T *G = a.Gb + off;     // G(c, i) = G[i*kb + c]
   T *Tg = a.Tb + off;
@@ -1601,13 +1611,14 @@ __global__ __launch_bounds__(256) void k_ref_copy(RefArgs r, double *H, const in
   for (size_t i = beg + threadIdx.x; i < end; i += 256) H[bd.offH + i] = r.X[bd.offZ + i];
 }
 
-// Concurrent column chains of the reduction: RED_GROUPS, or NRG_RED_STREAMS (1 .. 1 + NAUX) for
-// measurements
+// Concurrent column chains of the reduction: RED_GROUPS, or NRG_RED_STREAMS (1 .. NAUX) for
+// measurements (the caller's stream and NAUX - 1 extra streams; the last extra stream carries the
+// two-stage chain)
 static int red_groups() {
   static const int g = [] {
     const char *e = std::getenv("NRG_RED_STREAMS");
     const int v = e ? std::atoi(e) : RED_GROUPS;
-    return std::max(1, std::min(v, 1 + nrg_ctx::NAUX));
+    return std::max(1, std::min(v, nrg_ctx::NAUX));
   }();
   return g;
 }
@@ -1631,8 +1642,7 @@ int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, dou
       set_error("nrg_syev_batched: block order out of range [1, " + std::to_string(NMAX) + "]");
       return NRG_E_ARG;
     }
-  if (red_groups() > 1)
-    if (int r = ensure_aux_streams(ctx)) return r;
+  if (int r = ensure_aux_streams(ctx)) return r;
   if (mode == NRG_SYEV_MIXED) {
     extern int syev_mixed_run(nrg_ctx *, int, const int64_t *, const int64_t *, double *, const int64_t *,
                               double *, int *, hipStream_t);
@@ -1675,6 +1685,12 @@ struct SyevPlan {
   size_t g_items = 0, g_cnt = 0, yh_items = 0, yh_cnt = 0;
   std::vector<size_t> w_items, w_cnt, u_items, u_cnt;
   std::vector<size_t> s2_items, s2_cnt;   // trailing rank-2k updates per (panel slot, group): [q * ngrp + g]
+  // two-stage reduction of the blocks with two = 1
+  int nsb = 0, P1max = 0;                 // blocks reduced in two stages, their most stage-1 panels
+  int64_t sumAB = 0, sumV2 = 0, sumT2 = 0;
+  std::vector<size_t> sbp_off, sbp_cnt;   // stage-1 panel lists per panel slot (col_entry(b, q))
+  std::vector<size_t> sby_items, sby_cnt, sbu_items, sbu_cnt;   // Y = A22 V and the rank-2 SB update
+  size_t sbb_off = 0, sbb_cnt = 0, sbc_off = 0, sbc_cnt = 0, sbq_off = 0, sbq_cnt = 0;   // band, chase, Q2
 };
 
 // The work lists and product tables of a call, kept in the context for the next call: the lists of
@@ -1706,17 +1722,35 @@ static SyevCache &syev_cache(nrg_ctx *ctx) {
   return c;
 }
 
+// Back-transformation super-panels of a block: its reflectors (n - roff of them) in groups of KBT;
+// none for orders below 3 in one stage (the only reflector has length one).
+static int bt_panels(const BlkDesc &d) {
+  if (!d.two && d.n < 3) return 0;
+  return (int)((d.n - d.roff + KBT - 1) / KBT);
+}
+
 // Panels reduced by the column kernels: those that start with more than TAIL trailing rows. The
 // trailing matrix of order <= TAIL that remains is reduced by k_tail.
 static int main_panels(int64_t n) {
   return n > TAIL ? (int)((n - TAIL + NB - 1) / NB) : 0;
 }
 
+// Blocks reduced in two stages when NRG_TWO_STAGE=1 in the environment (read at every call): orders
+// above TAIL whose stage-1 panels fit one row per thread. Off by default: on MI355X the bulge chase's
+// critical path (each sweep three dependent steps behind the previous one, about 3n steps of a few
+// microseconds for a block of order n) exceeds the whole one-stage reduction at these orders.
+static bool two_stage_on() {
+  const char *e = std::getenv("NRG_TWO_STAGE");
+  return e && e[0] == '1';
+}
+static bool use_two_stage(int64_t n) { return two_stage_on() && n > TAIL && n <= SB_NMAX; }
+
 static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *off_H, const int64_t *off_w,
                        bool fp64_inplace, int elem_bytes, int ngrp) {
   pl.ngrp = std::max(1, std::min(ngrp, nblk));
   pl.bd.resize(nblk);
   int64_t zo = 0, vo = 0, wo = 0, ao = 0;
+  pl.nsb = 0; pl.P1max = 0; pl.sumAB = 0; pl.sumV2 = 0; pl.sumT2 = 0;
   for (int b = 0; b < nblk; b++) {
     BlkDesc &d = pl.bd[b];
     d.n = n[b];
@@ -1724,7 +1758,17 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
     d.offA = fp64_inplace ? off_H[b] : ao; ao += n[b] * n[b];
     d.offV = vo; vo += n[b];
     d.offWp = wo; wo += n[b] * NB;
-    d.P = main_panels(n[b]);
+    d.two = use_two_stage(n[b]) ? 1 : 0;
+    d.roff = d.two ? SB : 1;
+    d.offAB = pl.sumAB; d.offV2 = pl.sumV2; d.offT2 = pl.sumT2;
+    if (d.two) {
+      pl.nsb++;
+      pl.P1max = std::max(pl.P1max, sb_panels((int)n[b]));
+      pl.sumAB += 2 * SB * n[b];
+      pl.sumV2 += (int64_t)SB * n[b] * sb_ks((int)n[b]);
+      pl.sumT2 += n[b] * sb_ks((int)n[b]);
+    }
+    d.P = d.two ? 0 : main_panels(n[b]);
     d.offYp = pl.sumYp; pl.sumYp += n[b] * ((n[b] + 63) / 64);
     d.offDt = pl.sumYp; pl.sumYp += 64 * ((n[b] + 63) / 64);
     d.offH = off_H[b];
@@ -1830,11 +1874,34 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
         pl.cols.push_back(c);
       }
   }
-  // tail of every block (one k_tail launch after the column launches)
+  // tail of every block reduced in one stage (one k_tail launch after the column launches)
   pl.tail_off = I.size();
   for (int b : order)
-    if (n[b] >= 1) I.push_back(col_entry(b, pl.bd[b].P));
+    if (n[b] >= 1 && !pl.bd[b].two) I.push_back(col_entry(b, pl.bd[b].P));
   pl.tail_cnt = I.size() - pl.tail_off;
+  // two-stage reduction: stage-1 panels slot by slot (block b has panel q in slot q), the band
+  // copies, one bulge-chasing workgroup per block, and the stage-2 back-transformation slabs
+  pl.sbp_off.assign(pl.P1max, 0); pl.sbp_cnt.assign(pl.P1max, 0);
+  for (int q = 0; q < pl.P1max; q++) {
+    pl.sbp_off[q] = I.size();
+    for (int b : order)
+      if (pl.bd[b].two && sb_panels((int)n[b]) > q) I.push_back(col_entry(b, q));
+    pl.sbp_cnt[q] = I.size() - pl.sbp_off[q];
+  }
+  pl.sbb_off = I.size();
+  for (int b : order)
+    if (pl.bd[b].two)
+      for (int c = 0; c < (int)((n[b] + 63) / 64); c++) push2(b, c);
+  pl.sbb_cnt = (I.size() - pl.sbb_off) / 2;
+  pl.sbc_off = I.size();
+  for (int b : order)
+    if (pl.bd[b].two) I.push_back(b);
+  pl.sbc_cnt = I.size() - pl.sbc_off;
+  pl.sbq_off = I.size();
+  for (int b : order)
+    if (pl.bd[b].two)
+      for (int c = 0; c < (int)((n[b] + WAVE - 1) / WAVE); c++) push2(b, c);
+  pl.sbq_cnt = (I.size() - pl.sbq_off) / 2;
   // divide and conquer merges, deepest first
   pl.mrg_off.assign(pl.Lmax, 0); pl.mrg_cnt.assign(pl.Lmax, 0);
   pl.gemm_off.assign(pl.Lmax, 0); pl.gemm_cnt.assign(pl.Lmax, 0);
@@ -1889,7 +1956,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   pl.toff.assign(nblk, 0);
   pl.sumTb = 0; pl.sumW2 = 0; pl.Smax = 0;
   for (int b = 0; b < nblk; b++) {
-    const int sb = n[b] >= 3 ? (int)((n[b] - 1 + KBT - 1) / KBT) : 0;
+    const int sb = bt_panels(pl.bd[b]);
     pl.toff[b] = pl.sumTb;
     pl.sumTb += (int64_t)sb * KBT * KBT;
     pl.sumW2 += (int64_t)KBT * n[b];
@@ -1897,13 +1964,12 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   }
   pl.ym_off = I.size();
   for (int b : order)
-    if (n[b] >= 3)
-      for (int c = 0; c < (int)((n[b] - 1 + 63) / 64); c++) push2(b, c);
+    if (bt_panels(pl.bd[b]))
+      for (int c = 0; c < (int)((n[b] - pl.bd[b].roff + 63) / 64); c++) push2(b, c);
   pl.ym_cnt = (I.size() - pl.ym_off) / 2;
   pl.tb_off = I.size();
   for (int b : order)
-    if (n[b] >= 3)
-      for (int sp = 0; sp < (int)((n[b] - 1 + KBT - 1) / KBT); sp++) push2(b, sp);
+    for (int sp = 0; sp < bt_panels(pl.bd[b]); sp++) push2(b, sp);
   pl.tb_cnt = (I.size() - pl.tb_off) / 2;
   pl.cp_off = I.size();
   for (int b : order)
@@ -1951,9 +2017,40 @@ static void build_dc_gemms(SyevPlan &pl, double *Z0, double *Z1, double *S) {
 // work precision T; appends items to pl.ibuf.
 template <typename T>
 static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *Abase, T *Wpbase, T *Ym, T *Yh, T *Z,
-                               T *Gb, T *Tb, T *W2) {
+                               T *Gb, T *Tb, T *W2, T *sbV, T *sbY, T *sbW) {
   gd.clear();
   std::vector<int> &I = pl.ibuf;
+  // stage 1 of the two-stage reduction, per panel slot: Y = A22 V, then A22 -= V W^T + W V^T (every
+  // tile: the next panel's product reads the whole symmetric A22)
+  pl.sby_items.assign(pl.P1max, 0); pl.sby_cnt.assign(pl.P1max, 0);
+  pl.sbu_items.assign(pl.P1max, 0); pl.sbu_cnt.assign(pl.P1max, 0);
+  for (int q = 0; q < pl.P1max; q++)
+    for (int ph = 0; ph < 2; ph++) {
+      const size_t d0 = gd.size();
+      for (size_t b = 0; b < pl.bd.size(); b++) {
+        const BlkDesc &d = pl.bd[b];
+        if (!d.two || sb_panels((int)d.n) <= q) continue;
+        const int n = (int)d.n, k0 = q * SB, R = n - k0 - SB;
+        T *A22 = Abase + d.offA + (size_t)(k0 + SB) * n + (k0 + SB);
+        GemmDescT<T> g{};
+        if (ph == 0) {
+          g.A = A22; g.lda = n; g.B = sbV + d.offWp; g.ldb = n; g.K = R;
+          g.C = sbY + d.offWp; g.ldc = n;
+          g.M = R; g.N = SB; g.alpha = 1.0; g.beta = 0.0;
+        } else {
+          g.A = sbV + d.offWp; g.lda = n; g.B = sbW + d.offWp; g.ldb = n; g.tB = 1; g.K = SB;
+          g.A2 = sbW + d.offWp; g.lda2 = n; g.B2 = sbV + d.offWp; g.ldb2 = n; g.tB2 = 1; g.K2 = SB;
+          g.C = A22; g.ldc = n;
+          g.M = R; g.N = R; g.alpha = -1.0; g.beta = 1.0;
+        }
+        gd.push_back(g);
+      }
+      size_t &io = ph == 0 ? pl.sby_items[q] : pl.sbu_items[q];
+      size_t &ic = ph == 0 ? pl.sby_cnt[q] : pl.sbu_cnt[q];
+      io = I.size();
+      gemm_items(gd, d0, gd.size(), I);
+      ic = (I.size() - io) / 2;
+    }
   // trailing updates A22 -= V W^T + W V^T after every panel that is not a block's last
   pl.s2_items.assign((size_t)pl.Pmax * pl.ngrp, 0); pl.s2_cnt.assign((size_t)pl.Pmax * pl.ngrp, 0);
   for (int q = 0; q < pl.Pmax; q++)   // panel slot q, as in build_plan
@@ -1991,13 +2088,13 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
   }
   // back-transformation products
   const int nblk = (int)pl.bd.size();
-  auto nsp = [&](int b) { const int n = (int)pl.bd[b].n; return n >= 3 ? (n - 1 + KBT - 1) / KBT : 0; };
+  auto nsp = [&](int b) { return bt_panels(pl.bd[b]); };
   size_t d0 = gd.size();
   for (int b = 0; b < nblk; b++) {
     const BlkDesc &d = pl.bd[b];
     const int n = (int)d.n;
     for (int sp = 0; sp < nsp(b); sp++) {
-      const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
+      const int kb = std::min(KBT, (n - d.roff) - sp * KBT), r0 = sp * KBT + d.roff;
       GemmDescT<T> g{};
       g.A = Ym + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
       g.B = g.A; g.ldb = n; g.tB = 0;
@@ -2014,7 +2111,7 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
     const BlkDesc &d = pl.bd[b];
     const int n = (int)d.n;
     for (int sp = 0; sp < nsp(b); sp++) {
-      const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
+      const int kb = std::min(KBT, (n - d.roff) - sp * KBT), r0 = sp * KBT + d.roff;
       GemmDescT<T> g{};
       g.A = Ym + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 0;
       g.B = Tb + pl.toff[b] + (size_t)sp * KBT * KBT; g.ldb = kb; g.tB = 1;
@@ -2040,7 +2137,7 @@ static void build_red_bt_gemms(SyevPlan &pl, std::vector<GemmDescT<T>> &gd, T *A
         w2o += (int64_t)KBT * n;
         if (S_b <= step) continue;
         const int sp = S_b - 1 - step;
-        const int kb = std::min(KBT, (n - 1) - sp * KBT), r0 = sp * KBT + 1;
+        const int kb = std::min(KBT, (n - d.roff) - sp * KBT), r0 = sp * KBT + d.roff;
         GemmDescT<T> g{};
         if (phase == 0) {   // W = Yhat_s^T Z(r0:, :)
           g.A = Yh + d.offZ + (size_t)sp * KBT * n + r0; g.lda = n; g.tA = 1;
@@ -2092,13 +2189,30 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     NRG_TIMED(ctx, st, "k_absmax", hipLaunchKernelGGL(k_absmax<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off)));
   if (pl.symm_cnt)
     NRG_TIMED(ctx, st, "k_symm", hipLaunchKernelGGL(k_symm<T>, grid(pl.symm_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.symm_off), 0));
-  // 2. reduction: group g's chain on stream S[g], forked from and joined back into st
+  // 2. reduction: group g's chain on stream S[g], the two-stage chain on Ssb, forked from and joined
+  //    back into st
   const int NG = pl.ngrp;
   hipStream_t S[1 + nrg_ctx::NAUX] = {st};
   for (int g = 1; g < NG; g++) S[g] = ctx->aux[g - 1];
-  if (NG > 1) {
+  hipStream_t Ssb = ctx->aux[nrg_ctx::NAUX - 1];
+  if (NG > 1 || pl.nsb) {
     NRG_HIP(hipEventRecord(ctx->fork_ev, st));
     for (int g = 1; g < NG; g++) NRG_HIP(hipStreamWaitEvent(S[g], ctx->fork_ev, 0));
+    if (pl.nsb) NRG_HIP(hipStreamWaitEvent(Ssb, ctx->fork_ev, 0));
+  }
+  if (pl.nsb) {
+    for (int q = 0; q < pl.P1max; q++) {
+      const int *lst = I + pl.sbp_off[q];
+      NRG_TIMED(ctx, Ssb, "k_sb_panel", hipLaunchKernelGGL(k_sb_panel<T>, grid(pl.sbp_cnt[q]), dim3(NT_PANEL), 0, Ssb,
+                                                           aT, lst));
+      launch_gemm<T>(ctx, Ssb, L.dGT, I + pl.sby_items[q], H + pl.sby_items[q], pl.sby_cnt[q]);
+      NRG_TIMED(ctx, Ssb, "k_sb_w", hipLaunchKernelGGL(k_sb_w<T>, grid(pl.sbp_cnt[q]), dim3(NT_SBW), 0, Ssb, aT, lst));
+      launch_gemm<T>(ctx, Ssb, L.dGT, I + pl.sbu_items[q], H + pl.sbu_items[q], pl.sbu_cnt[q]);
+    }
+    NRG_TIMED(ctx, Ssb, "k_sb_toband", hipLaunchKernelGGL(k_sb_toband<T>, grid(pl.sbb_cnt), dim3(256), 0, Ssb, aT,
+                                                          (const int2 *)(I + pl.sbb_off)));
+    NRG_TIMED(ctx, Ssb, "k_sb_chase", hipLaunchKernelGGL(k_sb_chase<T>, grid(pl.sbc_cnt), dim3(SB_WAVES * WAVE), 0, Ssb,
+                                                         aT, I + pl.sbc_off));
   }
   size_t ci = 0;
   for (int q = 0; q < pl.Pmax; q++) {
@@ -2128,6 +2242,10 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
       NRG_HIP(hipEventRecord(ctx->join_ev[g - 1], S[g]));
       NRG_HIP(hipStreamWaitEvent(st, ctx->join_ev[g - 1], 0));
     }
+  if (pl.nsb) {
+    NRG_HIP(hipEventRecord(ctx->join_ev[nrg_ctx::NAUX - 1], Ssb));
+    NRG_HIP(hipStreamWaitEvent(st, ctx->join_ev[nrg_ctx::NAUX - 1], 0));
+  }
   if (pl.tail_cnt)
     NRG_TIMED(ctx, st, "k_tail", hipLaunchKernelGGL(k_tail<T>, grid(pl.tail_cnt), dim3(NT_PANEL), 0, st, aT, I + pl.tail_off));
   NRG_HIP(hipGetLastError());
@@ -2203,6 +2321,10 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   if (pl.tb_cnt)
     NRG_TIMED(ctx, st, "k_bt_tbig", hipLaunchKernelGGL(k_bt_tbig<T>, grid(pl.tb_cnt), dim3(256), 0, st, aT, (const int2 *)(I + pl.tb_off), L.toff));
   launch_gemm<T>(ctx, st, L.dGT, I + pl.yh_items, H + pl.yh_items, pl.yh_cnt);
+  // two-stage blocks: the bulge-chasing reflectors first (V = Q1 (Q2 Z))
+  if (pl.sbq_cnt)
+    NRG_TIMED(ctx, st, "k_sb_q2", hipLaunchKernelGGL(k_sb_q2<T>, grid(pl.sbq_cnt), dim3(WAVE), 0, st, aT,
+                                                    (const int2 *)(I + pl.sbq_off)));
   for (int step = 0; step < pl.Smax; step++) {
     launch_gemm<T>(ctx, st, L.dGT, I + pl.w_items[step], H + pl.w_items[step], pl.w_cnt[step]);
     launch_gemm<T>(ctx, st, L.dGT, I + pl.u_items[step], H + pl.u_items[step], pl.u_cnt[step]);
@@ -2254,12 +2376,17 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   const size_t ovb = cv.take<double>(pl.sumn), oYp = cv.take<double>(pl.sumYp);
   const size_t omst = cv.take<MergeState>(pl.mrg_max + 1), oamax = cv.take<double>(nblk);
   const size_t oGb = cv.take<double>(pl.sumTb + 1), oTb = cv.take<double>(pl.sumTb + 1), oW2 = cv.take<double>(pl.sumW2);
+  const size_t sbw = pl.nsb ? (size_t)pl.sumWp : 1;
+  const size_t osbV = cv.take<double>(sbw), osbY = cv.take<double>(sbw), osbW = cv.take<double>(sbw);
+  const size_t osbT = cv.take<double>(pl.nsb ? (size_t)nblk * SB * SB : 1), osbAB = cv.take<double>(pl.sumAB + 1);
+  const size_t osbV2 = cv.take<double>(pl.sumV2 + 1), osbT2 = cv.take<double>(pl.sumT2 + 1);
   if (int r = ctx->ws.ensure(cv.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
   build_dc_gemms(pl, Z0, Z1, S);
   std::vector<GemmDesc> &gdT = cache.g64;
   build_red_bt_gemms<double>(pl, gdT, H, ctx->ws.as<double>(oWp), S, Z1, Z0, ctx->ws.as<double>(oGb),
-                             ctx->ws.as<double>(oTb), ctx->ws.as<double>(oW2));
+                             ctx->ws.as<double>(oTb), ctx->ws.as<double>(oW2), ctx->ws.as<double>(osbV),
+                             ctx->ws.as<double>(osbY), ctx->ws.as<double>(osbW));
   // descriptor upload: [BlkDesc][GemmDesc D&C][GemmDesc work][toff][ints]
   const size_t bBD = sizeof(BlkDesc) * pl.bd.size(), bP = sizeof(GemmDesc) * pl.gd.size(), bI = sizeof(int) * pl.ibuf.size();
   const size_t bQ = sizeof(GemmDesc) * gdT.size(), bT = sizeof(int64_t) * pl.toff.size();
@@ -2298,6 +2425,9 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
   L.toff = (const int64_t *)(ctx->desc.as<char>(oT));
   a.Gb = ctx->ws.as<double>(oGb); a.Tb = ctx->ws.as<double>(oTb);
   a.Ym = S; a.Yh = Z1;
+  a.sbV = ctx->ws.as<double>(osbV); a.sbY = ctx->ws.as<double>(osbY); a.sbW = ctx->ws.as<double>(osbW);
+  a.sbT = ctx->ws.as<double>(osbT); a.sbAB = ctx->ws.as<double>(osbAB);
+  a.sbV2 = ctx->ws.as<double>(osbV2); a.sbT2 = ctx->ws.as<double>(osbT2);
   L.mst = ctx->ws.as<MergeState>(omst);
   if (int r = launch_core<double>(ctx, pl, a, a, L, st, true, 0)) return r;
   NRG_TIMED(ctx, st, "k_finish", hipLaunchKernelGGL(k_finish<double>, dim3((unsigned)nblk), dim3(256), 0, st, a, L.dI + pl.all_off));
@@ -2368,6 +2498,10 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   const size_t fvb = cf.take<float>(pl.sumn), fYp = cf.take<float>(pl.sumYp);
   const size_t fYm = cf.take<float>(pl.sumn2), fYh = cf.take<float>(pl.sumn2), fZt = cf.take<float>(pl.sumn2);
   const size_t fGb = cf.take<float>(pl.sumTb + 1), fTb = cf.take<float>(pl.sumTb + 1), fW2 = cf.take<float>(pl.sumW2);
+  const size_t sbw = pl.nsb ? (size_t)pl.sumWp : 1;
+  const size_t fsbV = cf.take<float>(sbw), fsbY = cf.take<float>(sbw), fsbW = cf.take<float>(sbw);
+  const size_t fsbT = cf.take<float>(pl.nsb ? (size_t)nblk * SB * SB : 1), fsbAB = cf.take<float>(pl.sumAB + 1);
+  const size_t fsbV2 = cf.take<float>(pl.sumV2 + 1), fsbT2 = cf.take<float>(pl.sumT2 + 1);
   if (int r = ctx->ws2.ensure(cf.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
   build_dc_gemms(pl, Z0, Z1, S);
@@ -2375,7 +2509,8 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   float *A32 = ctx->ws2.as<float>(fA);
   build_red_bt_gemms<float>(pl, gdT, A32, ctx->ws2.as<float>(fWp), ctx->ws2.as<float>(fYm), ctx->ws2.as<float>(fYh),
                             ctx->ws2.as<float>(fZt), ctx->ws2.as<float>(fGb), ctx->ws2.as<float>(fTb),
-                            ctx->ws2.as<float>(fW2));
+                            ctx->ws2.as<float>(fW2), ctx->ws2.as<float>(fsbV), ctx->ws2.as<float>(fsbY),
+                            ctx->ws2.as<float>(fsbW));
   // refinement products: AX = A X; S = X^T AX and R = I - X^T X; Xn = X (I + E). Both sweep
   // directions (X in Z0 -> Xn in Z1, then back) are prepared.
   std::vector<GemmDesc> &gref = cache.gref;
@@ -2447,6 +2582,9 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   aF.vb = ctx->ws2.as<float>(fvb); aF.Yp = ctx->ws2.as<float>(fYp);
   aF.Ym = ctx->ws2.as<float>(fYm); aF.Yh = ctx->ws2.as<float>(fYh); aF.Zt = ctx->ws2.as<float>(fZt);
   aF.Gb = ctx->ws2.as<float>(fGb); aF.Tb = ctx->ws2.as<float>(fTb);
+  aF.sbV = ctx->ws2.as<float>(fsbV); aF.sbY = ctx->ws2.as<float>(fsbY); aF.sbW = ctx->ws2.as<float>(fsbW);
+  aF.sbT = ctx->ws2.as<float>(fsbT); aF.sbAB = ctx->ws2.as<float>(fsbAB);
+  aF.sbV2 = ctx->ws2.as<float>(fsbV2); aF.sbT2 = ctx->ws2.as<float>(fsbT2);
   aF.Z0 = Z0; aF.Z1 = Z1; aF.S = S; aF.d = aD.d; aF.e = aD.e; aF.lam = aD.lam; aF.scale = aD.scale;
   aF.dscr = aD.dscr; aF.iscr = aD.iscr; aF.Ntot = pl.sumn; aF.flags = aD.flags; aF.amax = aD.amax; aF.H = H; aF.w = w;
   SyevLaunch L;

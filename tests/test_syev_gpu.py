@@ -235,3 +235,54 @@ def test_orders_above_4096(ctx, mode):
         res = float(np.max(np.abs(H @ V.T - V.T * w[None, :]))) / (hn * n * eps)
         log(f"n={n} mode={mode}: eig {eig:.2e}, orth ratio {orth:.2f}, residual ratio {res:.2f}")
         assert eig <= 1e-12 and orth <= 10 and res <= 10, (n, eig, orth, res)
+
+
+@pytest.fixture
+def two_stage(monkeypatch):
+    """The two-stage reduction (NRG_TWO_STAGE=1) for the blocks of orders 129 .. 1056."""
+    monkeypatch.setenv("NRG_TWO_STAGE", "1")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_two_stage_orders(ctx, mode, two_stage):
+    """Orders of the two-stage reduction (129 .. 1056: a band of half-width 32, then bulge chasing) at
+    its edges: the first and last two-stage orders, orders whose last stage-1 panel leaves 33 rows
+    (161, 193: a final reflector of length one), panel boundaries (255..257, 511..513, 1023..1025), and
+    1057, the first order above it (one-stage column reduction)."""
+    rng = np.random.default_rng(21)
+    orders = [129, 130, 161, 193, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025, 1055, 1056, 1057]
+    blocks = [workloads.eig_block(rng, n)[0] for n in orders]
+    check_against_oracle(ctx, blocks, mode, label="two-stage orders")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_two_stage_special_matrices(ctx, mode, two_stage):
+    """Structures the band reduction meets in NRG blocks, at two-stage orders: zero and identity
+    matrices, an already tridiagonal (diagonal) one, exactly block-diagonal blocks, a rank-one matrix,
+    a free chain and a matrix that is already a band of half-width 32 (every reflector of the
+    bulge chase of zero length or trivial)."""
+    rng = np.random.default_rng(22)
+    band = rng.standard_normal((300, 300))
+    band = np.triu(np.tril(band + band.T, 32))
+    blocks = [np.zeros((200, 200)), np.eye(201), np.diag(np.linspace(-3.0, 5.0, 300)),
+              np.triu(np.kron(np.eye(5), rng.standard_normal((50, 50)))),
+              np.triu(np.full((300, 300), 1.0)),
+              np.triu(np.diag(np.ones(399), 1) + np.diag(np.ones(399), -1)),
+              np.triu(band)]
+    check_against_oracle(ctx, blocks, mode, label="two-stage special")
+
+
+def test_two_stage_matches_one_stage(ctx, monkeypatch):
+    """The same batch reduced in two stages (NRG_TWO_STAGE=1) and in one (the default): both against
+    dsyev, and against each other through the eigenvalues."""
+    rng = np.random.default_rng(23)
+    blocks = [workloads.eig_block(rng, n)[0] for n in (140, 333, 700, 1000)]
+    sizes, off_H, arena, off_w = pack(blocks)
+    monkeypatch.setenv("NRG_TWO_STAGE", "1")
+    w2, _, _ = run_gpu(ctx, sizes, off_H, arena, off_w)
+    check_against_oracle(ctx, blocks, NRG_SYEV_FP64, label="two-stage")
+    monkeypatch.delenv("NRG_TWO_STAGE")
+    w1, _, _ = run_gpu(ctx, sizes, off_H, arena, off_w)
+    d = float(np.max(np.abs(w2 - w1))) / max(float(np.max(np.abs(w1))), 1e-300)
+    log(f"two-stage vs one-stage eigenvalues: {d:.2e}")
+    assert d <= 1e-13
