@@ -131,6 +131,9 @@ int ensure_aux_streams(nrg_ctx *ctx) {
     if (!ctx->aux[i]) NRG_HIP(hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
     if (!ctx->join_ev[i]) NRG_HIP(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
   }
+  NRG_HIP(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+  NRG_HIP(hipEventCreateWithFlags(&ctx->gfork, hipEventDisableTiming));
+  NRG_HIP(hipEventCreateWithFlags(&ctx->gjoin, hipEventDisableTiming));
   NRG_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
   return NRG_OK;
 }
@@ -187,7 +190,9 @@ int nrg_ctx_destroy(nrg_ctx *c) {
   if (!c) return NRG_OK;
   hipSetDevice(c->device);
   hipDeviceSynchronize();
+  c->syev_cache.reset();   // the eigensolver's graphs and their descriptor buffers
   c->ws.release();
+  c->dcbuf.release();
   c->ws2.release();
   c->desc.release();
   c->flags.release();
@@ -200,6 +205,9 @@ int nrg_ctx_destroy(nrg_ctx *c) {
     if (c->join_ev[i]) hipEventDestroy(c->join_ev[i]);
   }
   if (c->fork_ev) hipEventDestroy(c->fork_ev);
+  if (c->gstream) hipStreamDestroy(c->gstream);
+  if (c->gfork) hipEventDestroy(c->gfork);
+  if (c->gjoin) hipEventDestroy(c->gjoin);
   delete c;
   return NRG_OK;
 }

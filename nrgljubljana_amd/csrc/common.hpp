@@ -128,6 +128,10 @@ struct nrg_ctx {
   static constexpr int NAUX = 4;   // the last one carries the eigensolver's two-stage chain
   hipStream_t aux[NAUX] = {};
   hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {};
+  // stream on which the eigensolver captures and replays its graphs (the caller's stream may be the
+  // null stream, which cannot be captured), with its fork and join events
+  hipStream_t gstream = nullptr;
+  hipEvent_t gfork = nullptr, gjoin = nullptr;
   // the eigensolver's host-side work lists, kept between calls so that their memory is reused
   std::shared_ptr<void> syev_cache;
 };

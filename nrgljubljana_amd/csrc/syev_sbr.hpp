@@ -47,40 +47,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Sums of 16 values over a wave: afterwards lane l holds the sum of value (l >> 2) & 15 (a
-// transposed butterfly: each exchange halves the values a lane keeps).
-template <typename T>
-__device__ __forceinline__ T wave_sum16(T (&v)[16]) {
-  const int lane = threadIdx.x & (WAVE - 1);
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const bool hi = lane & 32;
-    const T send = hi ? v[i] : v[i + 8], keep = hi ? v[i + 8] : v[i];
-    v[i] = keep + __shfl_xor(send, 32, WAVE);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const bool hi = lane & 16;
-    const T send = hi ? v[i] : v[i + 4], keep = hi ? v[i + 4] : v[i];
-    v[i] = keep + __shfl_xor(send, 16, WAVE);
-  }
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    const bool hi = lane & 8;
-    const T send = hi ? v[i] : v[i + 2], keep = hi ? v[i + 2] : v[i];
-    v[i] = keep + __shfl_xor(send, 8, WAVE);
-  }
-  {
-    const bool hi = lane & 4;
-    const T send = hi ? v[0] : v[1], keep = hi ? v[1] : v[0];
-    v[0] = keep + __shfl_xor(send, 4, WAVE);
-  }
-  T s = v[0];
-  s += __shfl_xor(s, 2, WAVE);
-  s += __shfl_xor(s, 1, WAVE);
-  return s;
-}
-
 // ------------------------------------------------------------------------------------------
 // Stage 1, panel factorisation (dgeqr2 on the panel below the band + dlarft), item = (block, panel)
 // ------------------------------------------------------------------------------------------
@@ -138,7 +104,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_sb_panel(SyevArgs<T> a, const int 
           const int c = half * 16 + q;
           pv[q] = c > j ? v * x[c] : (c < j ? v * (tid > c ? x[c] : (tid == c ? (T)1 : (T)0)) : (T)0);
         }
-        const T ps = wave_sum16(pv);
+        const T ps = butterfly16(pv);   // lane l: the sum of value (l >> 2) & 15
         __syncthreads();   // red is free again (sig read, or the first half reduced)
         if ((lane & 3) == 0) red[wv * 16 + (lane >> 2)] = ps;
         __syncthreads();

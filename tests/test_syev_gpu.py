@@ -286,3 +286,52 @@ def test_two_stage_matches_one_stage(ctx, monkeypatch):
     d = float(np.max(np.abs(w2 - w1))) / max(float(np.max(np.abs(w1))), 1e-300)
     log(f"two-stage vs one-stage eigenvalues: {d:.2e}")
     assert d <= 1e-13
+
+
+def test_graph_replay_matches_eager(ctx, monkeypatch):
+    """With NRG_SYEV_GRAPH=1, repeated calls with the same block orders, offsets and arenas are
+    captured into a HIP graph on the second call and replayed afterwards. Every replay, with new
+    matrix contents in the same arena and other calls on the context in between (which rewrite its
+    shared descriptor buffer), gives bit for bit the eager result (NRG_SYEV_GRAPH=0) of the same
+    input."""
+    import torch
+    rng = np.random.default_rng(31)
+    sizes = np.array([300, 17, 129, 64, 700, 2, 1], np.int64)
+    off_H = np.zeros(len(sizes), np.int64)
+    off_w = np.zeros(len(sizes), np.int64)
+    off_H[1:] = np.cumsum(sizes[:-1] ** 2)
+    off_w[1:] = np.cumsum(sizes[:-1])
+    inputs = []
+    for seed in range(3):
+        r = np.random.default_rng(100 + seed)
+        inputs.append(np.concatenate([np.triu(workloads.eig_block(r, int(n))[0]).ravel() if n > 1 else r.standard_normal(1)
+                                      for n in sizes]))
+    other_sizes, other_offH, other_arena, other_offw = pack([workloads.eig_block(rng, n)[0] for n in (50, 90)])
+    H = torch.zeros(int(np.sum(sizes ** 2)), dtype=torch.float64, device="cuda")
+    w = torch.zeros(int(np.sum(sizes)), dtype=torch.float64, device="cuda")
+
+    def solve(x):
+        H.copy_(torch.from_numpy(x))
+        ctx.syev_batched(H, sizes, off_H, w, off_w)
+        torch.cuda.synchronize()
+        return w.cpu().numpy().copy(), H.cpu().numpy().copy()
+
+    for mode in (NRG_SYEV_FP64, NRG_SYEV_MIXED):
+        monkeypatch.setenv("NRG_SYEV_GRAPH", "0")
+
+        def solve_mode(x):
+            H.copy_(torch.from_numpy(x))
+            ctx.syev_batched(H, sizes, off_H, w, off_w, mode)
+            torch.cuda.synchronize()
+            return w.cpu().numpy().copy(), H.cpu().numpy().copy()
+
+        eager = [solve_mode(x) for x in inputs]
+        monkeypatch.setenv("NRG_SYEV_GRAPH", "1")
+        order = [0, 0, 1, 2, 0, 1, 2, 2]   # eager, capture, then replays
+        for k, which in enumerate(order):
+            wg, Vg = solve_mode(inputs[which])
+            assert np.array_equal(wg, eager[which][0]) and np.array_equal(Vg, eager[which][1]), (mode, k, which)
+            if k in (3, 5):   # other work on the context between replays
+                run_gpu(ctx, other_sizes, other_offH, other_arena, other_offw, mode)
+                torch.cuda.synchronize()
+        log(f"mode {mode}: {len(order)} calls (eager, capture, {len(order) - 2} replays) equal to the eager results")
