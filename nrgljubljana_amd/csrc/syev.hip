@@ -74,6 +74,9 @@ struct SyevArgs {
   int *flags;
   double *amax;    // per-block max-norm of the input (bit pattern updated with atomicMax)
   double *H, *w;
+  // column preparation of blocks with more than NT_PANEL trailing rows (k_col_yred)
+  T *cs;                // the reduced column after the panel terms, n per block
+  T *dpp;               // per 256-row chunk of the trailing rows: its part of (w^T v)
   // two-stage reduction (syev_sbr.hpp)
   T *sbV, *sbY, *sbW;   // explicit panel reflectors V, Y = A22 V and W, n x SB per block (ld n)
   T *sbT;               // T factor of each block's current panel, SB x SB
@@ -166,6 +169,7 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items, 
 //     memory round trips instead of one per dot product.
 // ------------------------------------------------------------------------------------------
 constexpr int KCH = 16;   // panel columns loaded per batch
+constexpr int NT_YR = 256;   // rows per workgroup of k_col_yred
 
 // RPT: rows per thread, 1 when every block of the launch has at most NT_PANEL rows [jl, m)
 template <typename T, int RPT>
@@ -189,7 +193,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   const int jp = jl - 1;                         // reflector whose W column is finished here
   const int tid = threadIdx.x;
   constexpr int NW = NT_PANEL / WAVE;
-  constexpr int KC = RPT > 1 ? KCH / 2 : KCH;   // fewer columns in flight when a thread owns several rows
+  constexpr int KC = KCH;
 
   __shared__ T part[NW][64];
   __shared__ T cwv[2 * NB];          // cw[k] = W(:,k)^T v and cv[k] = V(:,k)^T v over the trailing rows
@@ -199,7 +203,55 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
 
   const T tv = jl > 0 ? tau[jp] : (T)0;
   const int Tt = (m - jl + 63) / 64;             // 64-row segments of the rows [jl, m)
-  T v[RPT], y[RPT], c[RPT], aj[RPT];
+  T cf[RPT];   // column jl after its update, rows jl + tid + q NT_PANEL
+  if (m - jl > NT_PANEL) {
+    T v[RPT], y[RPT], aj[RPT];
+    T *c = cf;
+    // more rows than threads. The first column of a panel is only loaded. Otherwise k_col_yred has
+    // reduced the symmetric-product partials and applied the panel terms row by row in many
+    // workgroups: W(:,jp) holds tau (A22 v - V cw - W cv) and cs the column after the terms k < jp.
+    // Finish W(:,jp) with a2 v and the column with the term k = jp.
+    if (jl == 0) {
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int r = tid + q * NT_PANEL;
+        c[q] = r < m ? A[r] : (T)0;
+      }
+    } else {
+    const T *cs = a.cs + bd.offV + i0;
+    const T *dpp = a.dpp + bd.offV;
+    const int nch = (m - jl + NT_YR - 1) / NT_YR;
+    T dp = 0;
+    for (int q = 0; q < nch; q++) dp += dpp[q];
+    const T a2 = (T)(-0.5) * tv * dp;
+    if (tid == 0) sh[1] = do_col ? A[(size_t)jp * n + jl] : (T)0;   // V(jl, jp)
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int r = jl + tid + q * NT_PANEL;
+      v[q] = y[q] = c[q] = aj[q] = 0;
+      if (r < m) {
+        v[q] = vb[r];
+        y[q] = W[(size_t)jp * n + r] + a2 * v[q];
+        W[(size_t)jp * n + r] = y[q];
+        if (r == jl) sh[0] = y[q];
+        if (do_col) {
+          c[q] = cs[r];
+          aj[q] = A[(size_t)jp * n + r];
+        }
+      }
+    }
+    __syncthreads();
+    if (do_col) {
+      const T wj = sh[0], vj = sh[1];
+#pragma unroll
+      for (int q = 0; q < RPT; q++) c[q] -= aj[q] * wj + y[q] * vj;
+    }
+    __syncthreads();   // sh[1] is reused by the reflector below
+    }
+  } else {
+  // at most NT_PANEL rows [jl, m): one row per thread
+  constexpr int R1 = 1;
+  T v[R1], y[R1], c[R1], aj[R1];
   // 1. loads that depend on nothing computed here
   if (jl > 0) {
     const int cl = tid & 63;
@@ -213,7 +265,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     wrow[tid] = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
   }
 #pragma unroll
-  for (int q = 0; q < RPT; q++) {
+  for (int q = 0; q < R1; q++) {
     const int r = jl + tid + q * NT_PANEL;
     v[q] = y[q] = c[q] = aj[q] = 0;
     if (r < m) {
@@ -231,11 +283,11 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   // the first KP panel columns of the update, loaded with the loads above (none when a thread owns
   // several rows: registers). KP and NB are multiples of KC, so the batches below index the panel
   // arrays cwv, wrow, vrow below NB (columns past jp carry zeros)
-  constexpr int KP = RPT > 1 ? 0 : KC;
+  constexpr int KP = KC;
   static_assert(KP % KC == 0 && NB % KC == 0, "panel batches must tile the panel");
-  T ap[RPT][KP > 0 ? KP : 1], wp[RPT][KP > 0 ? KP : 1];
+  T ap[R1][KP], wp[R1][KP];
 #pragma unroll
-  for (int q = 0; q < RPT; q++) {
+  for (int q = 0; q < R1; q++) {
     const int r = jl + tid + q * NT_PANEL;
 #pragma unroll
     for (int k = 0; k < KP; k++) {
@@ -257,7 +309,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     //    of column jl: A(:,jl) -= V W(jl,:)^T + W V(jl,:)^T
     T dp = 0;
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
+    for (int q = 0; q < R1; q++) {
       const int r = jl + tid + q * NT_PANEL;
       if (r < m) {
         T ay = 0, ac = 0;
@@ -288,7 +340,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     }
     const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
+    for (int q = 0; q < R1; q++) {
       const int r = jl + tid + q * NT_PANEL;
       if (r < m) {
         y[q] += a2 * v[q];
@@ -300,14 +352,18 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     if (do_col) {
       const T wj = sh[0], vj = vrow[jp];
 #pragma unroll
-      for (int q = 0; q < RPT; q++) c[q] -= aj[q] * wj + y[q] * vj;
+      for (int q = 0; q < R1; q++) c[q] -= aj[q] * wj + y[q] * vj;
     }
+  }
+  cf[0] = c[0];
+#pragma unroll
+  for (int q = 1; q < RPT; q++) cf[q] = 0;
   }
   if (!do_col) return;
   if (jl == w) {   // last diagonal element of the block (row jl is owned by thread 0)
     if (tid == 0) {
-      A[(size_t)jl * n + jl] = c[0];
-      d[jl] = (double)c[0];
+      A[(size_t)jl * n + jl] = cf[0];
+      d[jl] = (double)cf[0];
     }
     return;
   }
@@ -316,8 +372,8 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     const int r = jl + tid + q * NT_PANEL;
-    if (r >= jl + 2 && r < m) nrm += c[q] * c[q];
-    if (r == jl + 1) sh[1] = c[q];
+    if (r >= jl + 2 && r < m) nrm += cf[q] * cf[q];
+    if (r == jl + 1) sh[1] = cf[q];
   }
   const T xn2 = block_sum<NT_PANEL>(nrm, red);
   const T alpha = sh[1];
@@ -336,13 +392,13 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     const int r = jl + tid + q * NT_PANEL;
     if (r < m) {
       if (r == jl) {
-        colj[r] = c[q];
-        d[jl] = (double)c[q];
+        colj[r] = cf[q];
+        d[jl] = (double)cf[q];
       } else if (r == jl + 1) {
         colj[r] = 1;
         vb[r] = 1;
       } else {
-        const T x = c[q] * scal;
+        const T x = cf[q] * scal;
         colj[r] = x;
         vb[r] = x;
       }
@@ -396,6 +452,92 @@ __device__ __forceinline__ T butterfly16(T (&x)[16]) {
   s += __shfl_xor(s, 2, WAVE);
   s += __shfl_xor(s, 1, WAVE);
   return s;
+}
+
+// ------------------------------------------------------------------------------------------
+// 2a'. the row-parallel part of column jl's preparation for blocks with more than NT_PANEL rows
+//     [jl, m): item (block entry, 256-row chunk). Reduces the symmetric-product partials of the
+//     previous reflector's row, applies the panel terms (V (W^T v) + W (V^T v) to the product, and
+//     V W(jl,:)^T + W V(jl,:)^T to column jl, k < jp), and leaves W(:,jp) = tau (A22 v - ...), the
+//     updated column in cs and the chunk's part of (W(:,jp)^T v) in dpp for k_col_prep, so that
+//     its one workgroup only touches a few vectors of the block's rows.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT_YR) void k_col_yred(SyevArgs<T> a, const int2 *items, int jl) {
+  const int2 it = items[blockIdx.x];
+  const int ent = it.x, p = ent >> COL_PANEL_SHIFT;
+  const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
+  const int n = (int)bd.n;
+  const int i0 = p * NB;
+  const int m = n - i0;
+  const T *A = a.A + bd.offA + (size_t)i0 * n + i0;
+  T *W = a.Wp + bd.offWp;
+  const T *yp = a.Yp + bd.offYp + i0;
+  const T *dt = a.Yp + bd.offDt;
+  const T *vb = a.vb + bd.offV + i0;
+  const bool final = (m - 1 <= NB);
+  const int w = final ? m - 1 : NB;
+  const bool do_col = jl < w || final;
+  const int jp = jl - 1;
+  const T tv = a.tau[bd.offV + i0 + jp];
+  const int Tt = (m - jl + 63) / 64;
+  constexpr int NW = NT_YR / WAVE;
+  __shared__ T part[NW][64];
+  __shared__ T cwv[2 * NB], vrow[NB], wrow[NB];
+  __shared__ T red[NW];
+  const int tid = threadIdx.x;
+  {
+    const int cl = tid & 63;
+    T s = 0;
+    if ((cl & 31) < jp)
+      for (int sg = tid >> 6; sg < Tt; sg += NW) s += dt[sg * 64 + cl];
+    part[tid >> 6][cl] = s;
+  }
+  if (tid < NB) {
+    vrow[tid] = (do_col && tid < jl) ? A[(size_t)tid * n + jl] : (T)0;
+    wrow[tid] = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
+  }
+  const int r = jl + it.y * NT_YR + tid;
+  const bool rin = r < m;
+  T y = 0, cc = 0, v = 0;
+  if (rin) {
+#pragma unroll 16
+    for (int sl = 0; sl < Tt; sl++) y += yp[(size_t)sl * n + r];
+    v = vb[r];
+    if (do_col) cc = A[(size_t)jl * n + r];
+  }
+  __syncthreads();
+  if (tid < 2 * NB) {
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) s += part[q][tid];
+    cwv[tid] = s;
+  }
+  __syncthreads();
+  T ay = 0, ac = 0;
+  if (rin)
+    for (int k0 = 0; k0 < jp; k0 += KCH) {
+      T ak[KCH], wk[KCH];
+#pragma unroll
+      for (int k = 0; k < KCH; k++) {
+        const bool in = k0 + k < jp;
+        ak[k] = in ? A[(size_t)(k0 + k) * n + r] : (T)0;
+        wk[k] = in ? W[(size_t)(k0 + k) * n + r] : (T)0;
+      }
+#pragma unroll
+      for (int k = 0; k < KCH; k++) {
+        const int kk = k0 + k < NB ? k0 + k : NB - 1;   // ak = wk = 0 past jp
+        ay += ak[k] * cwv[kk] + wk[k] * cwv[NB + kk];
+        ac += ak[k] * wrow[kk] + wk[k] * vrow[kk];
+      }
+    }
+  const T yy = (rin && tv != (T)0) ? tv * (y - ay) : (T)0;
+  if (rin) {
+    W[(size_t)jp * n + r] = yy;
+    if (do_col) a.cs[bd.offV + i0 + r] = cc - ac;
+  }
+  const T dps = block_sum<NT_YR>(yy * v, red);
+  if (tid == 0) a.dpp[bd.offV + it.y] = dps;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1664,7 +1806,7 @@ struct SyevPlan {
   std::vector<int> ibuf;
   size_t symm_off = 0, symm_cnt = 0;
   // reduced column jl of panel slot p for the blocks of group g
-  struct ColLaunch { int p, jl, g, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles; };
+  struct ColLaunch { int p, jl, g, rows; size_t prep_off, prep_cnt, symv_off, tpre_off, nact, tiles, yred_off, yred_cnt; };
   int ngrp = 1;
   std::vector<int> grp;   // group of every block
   std::vector<ColLaunch> cols;
@@ -1871,6 +2013,16 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
         for (size_t k = 0; k < np; k++) tmp.push_back(col_entry(mb[k], panel(mb[k])));
         c.prep_off = share(last_blk[g]);
         c.prep_cnt = np;
+        // row-parallel preparation (k_col_yred) of the blocks with more than NT_PANEL rows [jl, m)
+        c.yred_off = I.size();
+        if (jl > 0)
+          for (size_t k = 0; k < np; k++) {
+            const int b = mb[k];
+            const int64_t rows = n[b] - (int64_t)NB * panel(b) - jl;
+            if (rows > NT_PANEL)
+              for (int64_t ch = 0; ch < (rows + NT_YR - 1) / NT_YR; ch++) push2(col_entry(b, panel(b)), (int)ch);
+          }
+        c.yred_cnt = (I.size() - c.yred_off) / 2;
         if (ns == np) {
           c.symv_off = c.prep_off;
         } else {
@@ -2238,6 +2390,9 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     for (; ci < pl.cols.size() && pl.cols[ci].p == q; ci++) {
       const SyevPlan::ColLaunch &c = pl.cols[ci];
       hipStream_t sg = S[c.g];
+      if (c.yred_cnt)
+        NRG_TIMED(ctx, sg, "k_col_yred", hipLaunchKernelGGL(k_col_yred<T>, grid(c.yred_cnt), dim3(NT_YR), 0, sg, aT,
+                                                            (const int2 *)(I + c.yred_off), c.jl));
       if (c.rows <= NT_PANEL)
         NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, sg, aT,
                                                             I + c.prep_off, c.jl));
@@ -2539,6 +2694,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
     const size_t osbV = cv.take<double>(sbw), osbY = cv.take<double>(sbw), osbW = cv.take<double>(sbw);
     const size_t osbT = cv.take<double>(pl.nsb ? (size_t)nblk * SB * SB : 1), osbAB = cv.take<double>(pl.sumAB + 1);
     const size_t osbV2 = cv.take<double>(pl.sumV2 + 1), osbT2 = cv.take<double>(pl.sumT2 + 1);
+    const size_t ocs = cv.take<double>(pl.sumn), odpp = cv.take<double>(pl.sumn);
     if (int r = ctx->ws.ensure(cv.off)) return r;
     double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
     build_dc_gemms(pl, Z0, Z1, S);
@@ -2595,6 +2751,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
       a.sbV = ctx->ws.as<double>(osbV); a.sbY = ctx->ws.as<double>(osbY); a.sbW = ctx->ws.as<double>(osbW);
       a.sbT = ctx->ws.as<double>(osbT); a.sbAB = ctx->ws.as<double>(osbAB);
       a.sbV2 = ctx->ws.as<double>(osbV2); a.sbT2 = ctx->ws.as<double>(osbT2);
+      a.cs = ctx->ws.as<double>(ocs); a.dpp = ctx->ws.as<double>(odpp);
       L.mst = ctx->ws.as<MergeState>(omst);
       if ((lrc = launch_core<double>(ctx, pl, a, a, L, rs, true, 0))) break;
       NRG_TIMED(ctx, rs, "k_finish", hipLaunchKernelGGL(k_finish<double>, dim3((unsigned)nblk), dim3(256), 0, rs, a, L.dI + pl.all_off));
@@ -2681,6 +2838,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   const size_t fsbV = cf.take<float>(sbw), fsbY = cf.take<float>(sbw), fsbW = cf.take<float>(sbw);
   const size_t fsbT = cf.take<float>(pl.nsb ? (size_t)nblk * SB * SB : 1), fsbAB = cf.take<float>(pl.sumAB + 1);
   const size_t fsbV2 = cf.take<float>(pl.sumV2 + 1), fsbT2 = cf.take<float>(pl.sumT2 + 1);
+  const size_t fcs = cf.take<float>(pl.sumn), fdpp = cf.take<float>(pl.sumn);
   if (int r = ctx->ws2.ensure(cf.off)) return r;
   double *Z0 = ctx->ws.as<double>(oZ0), *Z1 = ctx->ws.as<double>(oZ1), *S = ctx->ws.as<double>(oS);
   build_dc_gemms(pl, Z0, Z1, S);
@@ -2772,6 +2930,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   aF.sbV = ctx->ws2.as<float>(fsbV); aF.sbY = ctx->ws2.as<float>(fsbY); aF.sbW = ctx->ws2.as<float>(fsbW);
   aF.sbT = ctx->ws2.as<float>(fsbT); aF.sbAB = ctx->ws2.as<float>(fsbAB);
   aF.sbV2 = ctx->ws2.as<float>(fsbV2); aF.sbT2 = ctx->ws2.as<float>(fsbT2);
+  aF.cs = ctx->ws2.as<float>(fcs); aF.dpp = ctx->ws2.as<float>(fdpp);
   aF.Z0 = Z0; aF.Z1 = Z1; aF.S = S; aF.d = aD.d; aF.e = aD.e; aF.lam = aD.lam; aF.scale = aD.scale;
   aF.dscr = aD.dscr; aF.iscr = aD.iscr; aF.Ntot = pl.sumn; aF.flags = aD.flags; aF.amax = aD.amax; aF.H = H; aF.w = w;
   SyevLaunch L;
