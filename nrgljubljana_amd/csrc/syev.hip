@@ -220,11 +220,9 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     } else {
     const T *cs = a.cs + bd.offV + i0;
     const T *dpp = a.dpp + bd.offV;
-    const int nch = (m - jl + NT_YR - 1) / NT_YR;
-    T dp = 0;
-    for (int q = 0; q < nch; q++) dp += dpp[q];
-    const T a2 = (T)(-0.5) * tv * dp;
+    const int nch = (m - jl + NT_YR - 1) / NT_YR;   // at most NMAX / NT_YR <= NT_PANEL chunks
     if (tid == 0) sh[1] = do_col ? A[(size_t)jp * n + jl] : (T)0;   // V(jl, jp)
+    const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(tid < nch ? dpp[tid] : (T)0, red);
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int r = jl + tid + q * NT_PANEL;
@@ -489,8 +487,10 @@ __global__ __launch_bounds__(NT_YR) void k_col_yred(SyevArgs<T> a, const int2 *i
   {
     const int cl = tid & 63;
     T s = 0;
-    if ((cl & 31) < jp)
+    if ((cl & 31) < jp) {
+#pragma unroll 8
       for (int sg = tid >> 6; sg < Tt; sg += NW) s += dt[sg * 64 + cl];
+    }
     part[tid >> 6][cl] = s;
   }
   if (tid < NB) {
@@ -501,8 +501,18 @@ __global__ __launch_bounds__(NT_YR) void k_col_yred(SyevArgs<T> a, const int2 *i
   const bool rin = r < m;
   T y = 0, cc = 0, v = 0;
   if (rin) {
-#pragma unroll 16
-    for (int sl = 0; sl < Tt; sl++) y += yp[(size_t)sl * n + r];
+    T y2 = 0;
+    int sl = 0;
+    for (; sl + 32 <= Tt; sl += 32) {   // 32 loads in flight, two accumulators
+#pragma unroll
+      for (int u = 0; u < 32; u += 2) {
+        y += yp[(size_t)(sl + u) * n + r];
+        y2 += yp[(size_t)(sl + u + 1) * n + r];
+      }
+    }
+#pragma unroll 8
+    for (; sl < Tt; sl++) y += yp[(size_t)sl * n + r];
+    y += y2;
     v = vb[r];
     if (do_col) cc = A[(size_t)jl * n + r];
   }
