@@ -169,6 +169,14 @@ def bench_eig(args):
         H.copy_(src, non_blocking=True)
         ctx.syev_batched(H, sizes, off_H, w, off_w, mode)
 
+    # the library's autotuner measures each candidate chain count on the first calls of this histogram
+    # signature: let it settle before the warm-up and the timed region
+    tuning_calls = 0
+    while tuning_calls < 12 and any(e["chains"] < 0 for e in ctx.syev_tuning().values()) or tuning_calls == 0:
+        step()
+        torch.cuda.synchronize()
+        tuning_calls += 1
+    tuning = ctx.syev_tuning()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -282,6 +290,7 @@ def bench_eig(args):
         "max_eig_err_rel": eig_err,
         "max_orth_err": orth_err,
         "kernels_ms_per_step": {k: round(v[0] / 2, 3) for k, v in kt.items()},
+        "autotune": autotune_summary(tuning, tuning_calls),
         "roofline": roofline,
         "roofline_dominant_kernel": roofdom,
         "roofline_other": others,
@@ -291,6 +300,20 @@ def bench_eig(args):
         out["cpu_baseline"] = cpu_baseline(sizes, off_H, arena, off_w, flops)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def autotune_summary(table, calls):
+    """The eigensolver autotuner's choice for the benchmark's signature and its gain over the fixed
+    default of two column chains (call times measured by the library on the tuning calls)."""
+    if not table:
+        return {"tuning_calls": calls, "table": table, "note": "fixed by NRG_RED_STREAMS or NRG_AUTOTUNE=0"}
+    sig, e = next(iter(table.items()))
+    ms = {int(k): v for k, v in e["ms"].items()}
+    out = {"tuning_calls": calls, "signature": sig, "chosen_chains": e["chains"],
+           "best_call_ms_per_candidate": e["ms"]}
+    if e["chains"] in ms and 2 in ms:
+        out["gain_over_default_2_chains"] = round(ms[2] / ms[e["chains"]], 4)
+    return out
 
 
 BF16_PEAK_TFLOPS = 2500.0
@@ -1354,7 +1377,8 @@ def bench_nrg(args):
                           for k, v in runs.items()},
            "largest_block": max(max(b) for b in tm["blocks"] if b),
            "blocks_per_step_max": max(len(b) for b in tm["blocks"]),
-           "mixed_vs_fp64_max_abs_diff": cmp, "cpu_baseline": cpu}
+           "mixed_vs_fp64_max_abs_diff": cmp,
+           "autotune": {k: v["timing"].get("syev_tuning") for k, v in runs.items()}, "cpu_baseline": cpu}
     shutil.rmtree(base, ignore_errors=True)
     print(json.dumps(out), flush=True)
 

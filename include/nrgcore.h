@@ -337,6 +337,16 @@ const char *nrg_phase_name(int phase);
  * nrg_kernel_time returns the summed milliseconds and launch count of one kernel (e.g. "k_panel")
  * since then; nrg_kernel_names writes the comma-separated names seen so far into buf.
  * No counterpart in the reference (its timers are the wall-clock sections of c++/time_mem.hpp). */
+/*
+ * The eigensolver's autotuner: the number of concurrent column chains of the reduction is chosen per
+ * coarse signature of a call's block-size histogram (form, octave of the largest order, octaves of the
+ * numbers of blocks above 1024 and in 129..1024) by timing each candidate on the first calls with that
+ * signature. nrg_syev_tuning writes the table as JSON text into buf: per signature the calls seen,
+ * the chosen count (-1 while measuring) and each measured candidate's call time in ms.
+ * No counterpart in the reference. NRG_RED_STREAMS fixes the count; NRG_AUTOTUNE=0 keeps the default.
+ */
+int nrg_syev_tuning(nrg_ctx *ctx, char *buf, int buflen);
+
 int nrg_kernel_timing(nrg_ctx *ctx, int enable);
 int nrg_kernel_time(nrg_ctx *ctx, const char *name, double *total_ms, long *launches);
 int nrg_kernel_names(nrg_ctx *ctx, char *buf, int buflen);

@@ -22,7 +22,7 @@ EXPORTED = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_version"
             "nrg_gemm3_grouped", "nrg_fdm_bin", "nrg_dmnrg_bin", "nrg_cfs_bin", "nrg_ft_bin", "nrg_fdm_mats", "nrg_dmnrg_mats",
             "nrg_ft_mats", "nrg_temp_sum", "nrg_temp_sum_noalloc", "nrg_broaden", "nrg_hamiltonian",
             "nrg_last_time_ms", "nrg_phase_name", "nrg_kernel_timing",
-            "nrg_kernel_time", "nrg_kernel_names"]
+            "nrg_kernel_time", "nrg_kernel_names", "nrg_syev_tuning"]
 
 
 NRG_TEMP_GT, NRG_TEMP_I1T, NRG_TEMP_I2T, NRG_TEMP_CHIT = 0, 1, 2, 3
@@ -129,6 +129,8 @@ def lib():
     L.nrg_kernel_time.restype = ctypes.c_int
     L.nrg_kernel_names.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.nrg_kernel_names.restype = ctypes.c_int
+    L.nrg_syev_tuning.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.nrg_syev_tuning.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -318,6 +320,14 @@ class Context:
             lib().nrg_kernel_time(self._h, name.encode(), ctypes.byref(t), ctypes.byref(c))
             out[name] = (t.value, c.value)
         return out
+
+    def syev_tuning(self):
+        """The eigensolver autotuner's table: {signature: {"calls", "chains", "ms": {chains: call ms}}}."""
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        if lib().nrg_syev_tuning(self._h, buf, 1 << 16) != 0:
+            raise NrgError(-1, "nrg_syev_tuning")
+        return json.loads(buf.value.decode())
 
     def last_time_ms(self, phase=0):
         return lib().nrg_last_time_ms(self._h, int(phase))

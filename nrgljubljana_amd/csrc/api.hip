@@ -424,6 +424,20 @@ int nrg_kernel_names(nrg_ctx *ctx, char *buf, int buflen) {
   return NRG_OK;
 }
 
+int nrg_syev_tuning(nrg_ctx *ctx, char *buf, int buflen) {
+  if (!ctx || !buf || buflen <= 0) {
+    set_error("nrg_syev_tuning: invalid argument");
+    return NRG_E_ARG;
+  }
+  const std::string s = syev_tuning_json(ctx);
+  if ((int)s.size() >= buflen) {
+    set_error("nrg_syev_tuning: buffer too small");
+    return NRG_E_ARG;
+  }
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return NRG_OK;
+}
+
 const char *nrg_phase_name(int phase) {
   static const char *names[] = {"total",          "reduce (tridiagonal / dense products)",
                                 "tridiagonal eigensolver (divide and conquer)",

@@ -143,6 +143,7 @@ int scratch_setup(int device);
 int ensure_aux_streams(nrg_ctx *ctx);
 int syev_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
              double *w, int mode, int *info, hipStream_t st);
+std::string syev_tuning_json(nrg_ctx *ctx);
 int gemm3_run(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, int precision, hipStream_t st);
 int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf,
             int sign, double *gt_pos, double *gt_neg, double *ls_pos, double *ls_neg, hipStream_t st);

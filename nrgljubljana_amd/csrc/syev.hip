@@ -27,6 +27,8 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cstring>
 #include <cstdlib>
 #include <cmath>
@@ -1763,16 +1765,11 @@ __global__ __launch_bounds__(256) void k_ref_copy(RefArgs r, double *H, const in
   for (size_t i = beg + threadIdx.x; i < end; i += 256) H[bd.offH + i] = r.X[bd.offZ + i];
 }
 
-// Concurrent column chains of the reduction: RED_GROUPS, or NRG_RED_STREAMS (1 .. NAUX) for
-// measurements (the caller's stream and NAUX - 1 extra streams; the last extra stream carries the
-// two-stage chain)
-static int red_groups() {
-  static const int g = [] {
-    const char *e = std::getenv("NRG_RED_STREAMS");
-    const int v = e ? std::atoi(e) : RED_GROUPS;
-    return std::max(1, std::min(v, nrg_ctx::NAUX));
-  }();
-  return g;
+// Concurrent column chains of the reduction fixed by NRG_RED_STREAMS (1 .. NAUX: the caller's stream
+// and NAUX - 1 extra streams; the last extra stream carries the two-stage chain), or 0 when unset.
+static int red_groups_env() {
+  const char *e = std::getenv("NRG_RED_STREAMS");
+  return e ? std::max(1, std::min(std::atoi(e), nrg_ctx::NAUX)) : 0;
 }
 
 // ==========================================================================================
@@ -1862,8 +1859,29 @@ struct SyevGraph {
   }
 };
 
+// Autotuner of the number of concurrent column chains. A call is described by a coarse signature of
+// its block-size histogram: the form, the octave of the largest order, and the octaves of the numbers
+// of blocks above 1024 and in 129..1024. The first call with a signature runs the default and is not
+// measured (it pays for first-use costs); the next TUNE_ROUNDS x NTUNE calls run the candidates
+// (TUNE_CAND chains) in turn, each measured by the call's own event time per unit of work (Σ n³, so
+// that the different steps of an NRG run with one signature compare); then the candidate with the
+// smallest time per unit is used for that signature. NRG_RED_STREAMS fixes the count, NRG_AUTOTUNE=0
+// keeps RED_GROUPS.
+constexpr int TUNE_CAND[] = {2, 3, 4};
+constexpr int NTUNE = sizeof(TUNE_CAND) / sizeof(int);
+constexpr int TUNE_ROUNDS = 2;
+struct TuneEntry {
+  double unit[NTUNE] = {};   // least ms per 1e9 n^3 measured for each candidate
+  float ms[NTUNE] = {};      // call time of that measurement
+  int step = -1;             // next measurement (-1: the unmeasured first call; then k = round * NTUNE + cand)
+  int best = -1;             // chosen candidate once every measurement has run
+  long calls = 0;
+};
+using TuneSig = std::array<int, 4>;
+
 struct SyevCache {
   SyevPlan pl;
+  std::map<TuneSig, TuneEntry> tune;
   std::vector<GemmDesc> g64, gref;
   std::vector<GemmDesc32> g32;
   // graph replay of repeated calls: the key of the previous call, and the graph of the last key
@@ -1891,6 +1909,56 @@ static SyevCache &syev_cache(nrg_ctx *ctx) {
   c.gref.clear();
   c.g32.clear();
   return c;
+}
+
+static bool autotune_on() {
+  const char *e = std::getenv("NRG_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
+static TuneSig tune_signature(int form, int nblk, const int64_t *n) {
+  int64_t nmax = 1, big = 0, mid = 0;
+  for (int b = 0; b < nblk; b++) {
+    nmax = std::max(nmax, n[b]);
+    big += n[b] > 1024;
+    mid += n[b] > TAIL && n[b] <= 1024;
+  }
+  auto oct = [](int64_t x) { return x > 0 ? 64 - __builtin_clzll((unsigned long long)x) : 0; };
+  return TuneSig{form, oct(nmax), oct(big), oct(mid)};
+}
+
+// The chain count of this call: fixed by the environment, the tuner's choice, or what the tuner
+// measures next (*cand >= 0: the call's time is recorded as measurement *cand).
+static int tune_choose(SyevCache &c, const TuneSig &sig, int *cand) {
+  *cand = -1;
+  if (const int g = red_groups_env()) return g;
+  if (!autotune_on()) return RED_GROUPS;
+  TuneEntry &t = c.tune[sig];
+  t.calls++;
+  if (t.best >= 0) return TUNE_CAND[t.best];
+  *cand = t.step;
+  return t.step < 0 ? RED_GROUPS : TUNE_CAND[t.step % NTUNE];
+}
+
+static void tune_record(SyevCache &c, const TuneSig &sig, int cand, float ms, int nblk, const int64_t *n) {
+  if (cand < -1 || !c.tune.count(sig)) return;
+  TuneEntry &t = c.tune[sig];
+  if (t.best >= 0 || cand != t.step || !(ms > 0)) return;
+  if (cand >= 0) {
+    double n3 = 0;
+    for (int b = 0; b < nblk; b++) n3 += (double)n[b] * n[b] * n[b];
+    const double u = ms / (n3 * 1e-9);
+    const int k = cand % NTUNE;
+    if (t.unit[k] == 0 || u < t.unit[k]) {
+      t.unit[k] = u;
+      t.ms[k] = ms;
+    }
+  }
+  if (++t.step == TUNE_ROUNDS * NTUNE) {
+    t.best = 0;
+    for (int k = 1; k < NTUNE; k++)
+      if (t.unit[k] < t.unit[t.best]) t.best = k;
+  }
 }
 
 // Back-transformation super-panels of a block: its reflectors (n - roff of them) in groups of KBT;
@@ -2544,10 +2612,10 @@ static bool graphs_on() {
 
 static std::vector<int64_t> syev_key(nrg_ctx *ctx, int form, int nblk, const int64_t *n, const int64_t *off_H,
                                      const int64_t *off_w, const double *H, const double *w, hipStream_t st,
-                                     int setting) {
+                                     int setting, int groups) {
   std::vector<int64_t> k;
   k.reserve(3 * (size_t)nblk + 16);
-  k.push_back(form); k.push_back(nblk); k.push_back(red_groups()); k.push_back(two_stage_on()); k.push_back(setting);
+  k.push_back(form); k.push_back(nblk); k.push_back(groups); k.push_back(two_stage_on()); k.push_back(setting);
   for (const void *q : {(const void *)H, (const void *)w, (const void *)st, (const void *)ctx->ws.p,
                         (const void *)ctx->ws2.p, (const void *)ctx->dcbuf.p, (const void *)ctx->flags.p})
     k.push_back((int64_t)(intptr_t)q);
@@ -2682,16 +2750,23 @@ static int graph_capture_end(nrg_ctx *ctx, SyevCache &c, const std::vector<int64
 int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                   double *w, int *info, hipStream_t st) {
   SyevCache &cache = syev_cache(ctx);
+  const TuneSig sig = tune_signature(NRG_SYEV_FP64, nblk, n);
+  int cand = -1;
+  const int groups = tune_choose(cache, sig, &cand);
   const bool gon = graphs_on() && !ctx->kt.on && !cache.graph_failed;
   std::vector<int64_t> key;
-  if (gon) key = syev_key(ctx, NRG_SYEV_FP64, nblk, n, off_H, off_w, H, w, st, 0);
+  if (gon) key = syev_key(ctx, NRG_SYEV_FP64, nblk, n, off_H, off_w, H, w, st, 0, groups);
   bool replay = false;
   const GraphUse gu = graph_begin(ctx, cache, key, gon, &replay);
-  if (replay) return graph_replay(ctx, cache, nblk, info, st);
+  if (replay) {
+    const int r = graph_replay(ctx, cache, nblk, info, st);
+    tune_record(cache, sig, cand, ctx->total_ms, nblk, n);
+    return r;
+  }
   for (int attempt = 0; attempt < 2; attempt++) {
     const bool capture = gu == GraphUse::capture && attempt == 0;
     SyevPlan &pl = cache.pl;
-    build_plan(pl, nblk, n, off_H, off_w, true, 8, red_groups());
+    build_plan(pl, nblk, n, off_H, off_w, true, 8, groups);
     Carver cv;
     const size_t oZ0 = cv.take<double>(pl.sumn2), oZ1 = cv.take<double>(pl.sumn2), oS = cv.take<double>(pl.sumn2);
     const size_t oWp = cv.take<double>(pl.sumWp), oTau = cv.take<double>(pl.sumn);
@@ -2771,7 +2846,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
     } while (false);
     if (capture) {
       bool retry = false;
-      const std::vector<int64_t> key_now = syev_key(ctx, NRG_SYEV_FP64, nblk, n, off_H, off_w, H, w, st, 0);
+      const std::vector<int64_t> key_now = syev_key(ctx, NRG_SYEV_FP64, nblk, n, off_H, off_w, H, w, st, 0, groups);
       if (int r = graph_capture_end(ctx, cache, key_now, lrc, 4, st, &retry)) return r;
       if (retry) continue;
     } else if (lrc) {
@@ -2780,6 +2855,7 @@ int syev_fp64_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H
     const int r = finish_flags(ctx, nblk, info, st);
     if (ctx->kt.on) ctx->kt.collect();
     record_phase_times(ctx, 4);
+    if (!ctx->kt.on) tune_record(cache, sig, cand, ctx->total_ms, nblk, n);
     return r;
   }
   set_error("nrg_syev_batched: graph capture failed");
@@ -2807,16 +2883,23 @@ static int refine_sweeps() {
 int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_H, double *H, const int64_t *off_w,
                    double *w, int *info, hipStream_t st) {
   SyevCache &cache = syev_cache(ctx);
+  const TuneSig sig = tune_signature(NRG_SYEV_MIXED, nblk, n);
+  int cand = -1;
+  const int groups = tune_choose(cache, sig, &cand);
   const bool gon = graphs_on() && !ctx->kt.on && !cache.graph_failed;
   std::vector<int64_t> key;
-  if (gon) key = syev_key(ctx, NRG_SYEV_MIXED, nblk, n, off_H, off_w, H, w, st, refine_sweeps());
+  if (gon) key = syev_key(ctx, NRG_SYEV_MIXED, nblk, n, off_H, off_w, H, w, st, refine_sweeps(), groups);
   bool replay = false;
   const GraphUse gu = graph_begin(ctx, cache, key, gon, &replay);
-  if (replay) return graph_replay(ctx, cache, nblk, info, st);
+  if (replay) {
+    const int r = graph_replay(ctx, cache, nblk, info, st);
+    tune_record(cache, sig, cand, ctx->total_ms, nblk, n);
+    return r;
+  }
   for (int attempt = 0; attempt < 2; attempt++) {
   const bool capture = gu == GraphUse::capture && attempt == 0;
   SyevPlan &pl = cache.pl;
-  build_plan(pl, nblk, n, off_H, off_w, false, 4, red_groups());
+  build_plan(pl, nblk, n, off_H, off_w, false, 4, groups);
   // per-block chunk items for the elementwise refinement kernels
   const size_t rc_off = pl.ibuf.size();
   for (int b = 0; b < nblk; b++)
@@ -2993,7 +3076,7 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   } while (false);
   if (capture) {
     bool retry = false;
-    const std::vector<int64_t> key_now = syev_key(ctx, NRG_SYEV_MIXED, nblk, n, off_H, off_w, H, w, st, refine_sweeps());
+    const std::vector<int64_t> key_now = syev_key(ctx, NRG_SYEV_MIXED, nblk, n, off_H, off_w, H, w, st, refine_sweeps(), groups);
     if (int r = graph_capture_end(ctx, cache, key_now, lrc, 5, st, &retry)) return r;
     if (retry) continue;
   } else if (lrc) {
@@ -3002,10 +3085,43 @@ int syev_mixed_run(nrg_ctx *ctx, int nblk, const int64_t *n, const int64_t *off_
   const int r = finish_flags(ctx, nblk, info, st);
   if (ctx->kt.on) ctx->kt.collect();
   record_phase_times(ctx, 5);
+  if (!ctx->kt.on) tune_record(cache, sig, cand, ctx->total_ms, nblk, n);
   return r;
   }
   set_error("nrg_syev_batched: graph capture failed");
   return NRG_E_HIP;
+}
+
+// The tuner's table as JSON text: one entry per signature with its candidates' times and choice.
+std::string syev_tuning_json(nrg_ctx *ctx) {
+  std::string out = "{";
+  if (ctx->syev_cache) {
+    const SyevCache &c = *static_cast<SyevCache *>(ctx->syev_cache.get());
+    bool first = true;
+    for (const auto &[sig, t] : c.tune) {
+      char buf[1024];
+      int len = snprintf(buf, sizeof buf, "%s\"%s maxn<2^%d big<2^%d mid<2^%d\": {\"calls\": %ld, \"chains\": %d, \"ms\": {",
+                         first ? "" : ", ", sig[0] == NRG_SYEV_MIXED ? "mixed" : "fp64", sig[1], sig[2], sig[3], t.calls,
+                         t.best >= 0 ? TUNE_CAND[t.best] : -1);
+      bool f2 = true;
+      for (int k = 0; k < NTUNE && len < (int)sizeof buf - 64; k++)
+        if (t.ms[k] > 0) {
+          len += snprintf(buf + len, sizeof buf - len, "%s\"%d\": %.3f", f2 ? "" : ", ", TUNE_CAND[k], t.ms[k]);
+          f2 = false;
+        }
+      len += snprintf(buf + len, sizeof buf - len, "}, \"ms_per_1e9_n3\": {");
+      f2 = true;
+      for (int k = 0; k < NTUNE && len < (int)sizeof buf - 64; k++)
+        if (t.unit[k] > 0) {
+          len += snprintf(buf + len, sizeof buf - len, "%s\"%d\": %.4f", f2 ? "" : ", ", TUNE_CAND[k], t.unit[k]);
+          f2 = false;
+        }
+      snprintf(buf + len, sizeof buf - len, "}}");
+      out += buf;
+      first = false;
+    }
+  }
+  return out + "}";
 }
 
 }  // namespace nrg

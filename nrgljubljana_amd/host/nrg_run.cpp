@@ -23,7 +23,7 @@ std::string library_dir() {
   return ".";
 }
 
-void write_timing(const std::string &dir, const NRG::Timing &t) {
+void write_timing(const std::string &dir, const NRG::Timing &t, nrg_ctx *ctx) {
   std::ofstream F(dir + "/nrg_timing.json");
   F << std::setprecision(9) << "{\"sections\": {";
   bool first = true;
@@ -41,7 +41,11 @@ void write_timing(const std::string &dir, const NRG::Timing &t) {
     for (size_t k = 0; k < t.blocks[i].size(); k++) F << (k ? ", " : "") << t.blocks[i][k];
     F << "]";
   }
-  F << "]}\n";
+  F << "]";
+  // the eigensolver autotuner's choices over the run's block histograms
+  std::vector<char> buf(1 << 16);
+  if (ctx && nrg_syev_tuning(ctx, buf.data(), (int)buf.size()) == NRG_OK) F << ", \"syev_tuning\": " << buf.data();
+  F << "}\n";
 }
 }  // namespace
 
@@ -59,7 +63,7 @@ extern "C" int nrg_run_dir(const char *dir, const char *tables_dir, int device, 
       NRG::Device dev(device);
       NRG::NRGCalculation calc(dir, tables, dev, log);
       calc.run();
-      write_timing(dir, calc.timing());
+      write_timing(dir, calc.timing(), dev.ctx());
     }
     NRG::DevicePool::get().trim();   // every device buffer of the run has been released
     return 0;

@@ -316,6 +316,7 @@ def test_graph_replay_matches_eager(ctx, monkeypatch):
         torch.cuda.synchronize()
         return w.cpu().numpy().copy(), H.cpu().numpy().copy()
 
+    monkeypatch.setenv("NRG_AUTOTUNE", "0")   # one chain count for every call: the key repeats
     for mode in (NRG_SYEV_FP64, NRG_SYEV_MIXED):
         monkeypatch.setenv("NRG_SYEV_GRAPH", "0")
 
@@ -335,3 +336,34 @@ def test_graph_replay_matches_eager(ctx, monkeypatch):
                 run_gpu(ctx, other_sizes, other_offH, other_arena, other_offw, mode)
                 torch.cuda.synchronize()
         log(f"mode {mode}: {len(order)} calls (eager, capture, {len(order) - 2} replays) equal to the eager results")
+
+
+def test_autotuner_measures_then_chooses(ctx, monkeypatch):
+    """For a batch's signature the first call is not measured, then each candidate chain count is
+    measured twice, then the fastest per unit of work is used; results are identical whatever the count
+    (the chains only group the blocks differently)."""
+    import torch
+    monkeypatch.delenv("NRG_RED_STREAMS", raising=False)
+    monkeypatch.setenv("NRG_AUTOTUNE", "1")
+    rng = np.random.default_rng(41)
+    blocks = [workloads.eig_block(rng, n)[0] for n in (1100, 600, 300, 150, 90, 40)]
+    sizes, off_H, arena, off_w = pack(blocks)
+    from nrgljubljana_amd import Context
+    c = Context(0)
+    results = []
+    for k in range(9):
+        H = torch.from_numpy(arena.copy()).cuda()
+        w = torch.zeros(int(np.sum(sizes)), dtype=torch.float64, device="cuda")
+        c.syev_batched(H, sizes, off_H, w, off_w)
+        torch.cuda.synchronize()
+        results.append((w.cpu().numpy(), H.cpu().numpy()))
+    table = c.syev_tuning()
+    c.close()
+    log(f"tuning table: {table}")
+    assert len(table) == 1
+    (entry,) = table.values()
+    assert entry["calls"] == 9 and entry["chains"] in (2, 3, 4) and len(entry["ms_per_1e9_n3"]) == 3
+    unit = entry["ms_per_1e9_n3"]
+    assert unit[str(entry["chains"])] == min(unit.values())
+    for wk, Vk in results[1:]:
+        assert np.array_equal(wk, results[0][0]) and np.array_equal(Vk, results[0][1])
