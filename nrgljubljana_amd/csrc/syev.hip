@@ -52,7 +52,7 @@ inline int panel_in_slot(int q, int P) { return q < P ? q : -1; }
 struct BlkDesc {
   int64_t n, offA, offZ, offV, offWp, offH, offw, offYp, offDt;
   int64_t offAB, offV2, offT2;   // two-stage reduction: band, bulge-chasing reflectors and scalars
-  int L, P;   // P: panels reduced by the column kernels (before the tail)
+  int L, P;   // P: panels reduced by the column kernels (before k_wg_red and the tail)
   int two;    // 1: reduced in two stages (dense -> band -> tridiagonal)
   int roff;   // reflector j of the reduction starts at row j + roff (1, or SB in two stages)
 };
@@ -637,6 +637,224 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
   }
   __syncthreads();
   if (tid < 64 && rin) yp[(size_t)J * n + r] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+}
+
+// ------------------------------------------------------------------------------------------
+// 2d. one workgroup per block for trailing orders up to WGR_MAX: the blocked Householder reduction
+//     (dlatrd, then the rank-2k update) of every panel that starts with more than TAIL rows, with
+//     the matrix in global memory (a few MB: L2-resident). A thread owns a row of the trailing
+//     matrix and keeps the row's V and W in registers; panels here are KW columns wide (the
+//     column kernels' NB-column panels, as two halves: W is internal, the reflectors are stored as
+//     the column kernels store them). Each column's symmetric product runs over 64x64 lower tiles
+//     of the trailing matrix, one wave per tile, into per-wave partial sums in LDS (deterministic);
+//     each panel's trailing update runs over the lower triangle, a column tile at a time with the
+//     tile's V and W rows in LDS. It replaces, for these orders, the chain of about 33 launches per
+//     panel of the column kernels, and blocks of order up to WGR_MAX run beside the larger blocks'
+//     chains instead of in them. Opt-in (NRG_WG_RED): slower than the chains on MI355X.
+// ------------------------------------------------------------------------------------------
+// Raw buffer loads: one 32-bit per-lane offset register shared by many loads whose uniform parts
+// sit in scalar registers. Callers keep every offset inside the buffer.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+
+constexpr int WGR_MAX = 512;          // largest trailing order of k_wg_red: one row per thread
+constexpr int NW_WGR = WGR_MAX / WAVE;
+constexpr int KW = 16;                // panel width of k_wg_red
+static_assert(NB % KW == 0, "k_wg_red panels must tile the column kernels' panels");
+
+template <typename T>
+__global__ __launch_bounds__(WGR_MAX) void k_wg_red(SyevArgs<T> a, const int *blist) {
+  const int ent = blist[blockIdx.x], p0 = ent >> COL_PANEL_SHIFT;
+  const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
+  const int n = (int)bd.n;
+  const int iend = n > TAIL ? (n - TAIL + NB - 1) / NB * NB : 0;   // k_tail starts at row iend
+  // the wave index, in a scalar register: tile indices and load offsets derived from it are uniform
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
+  __shared__ T ypart[NW_WGR][WGR_MAX];   // per-wave partial sums of A22 v
+  __shared__ T vsh[WGR_MAX + 64];        // the current reflector (panel-relative rows), zero padded
+  __shared__ T rowj[2 * KW];             // V(jl, :) and W(jl, :)
+  __shared__ T dots[NW_WGR][2 * KW];     // per-wave partials of W^T v and V^T v
+  __shared__ T cwv[2 * KW];              // W^T v (0..KW) and V^T v (KW..2KW)
+  __shared__ T red[NW_WGR];
+  __shared__ T sh[2];
+  __shared__ T tV[64 * KW], tW[64 * KW]; // V and W rows of a column tile of the trailing update
+  const int r = tid;                     // this thread's row of the panel
+  if (tid < 64) vsh[WGR_MAX + tid] = 0;
+  for (int i0 = p0 * NB; i0 < iend; i0 += KW) {
+    const int m = n - i0;                // TAIL < m <= WGR_MAX
+    T *A = a.A + bd.offA + (size_t)i0 * n + i0;   // A(r,c) = A[c*n + r], relative to the panel
+    T *W = a.Wp + bd.offWp;                        // W(r,k) = W[k*n + r]
+    double *d = a.d + bd.offV + i0, *e = a.e + bd.offV + i0;
+    T *tau = a.tau + bd.offV + i0;
+    const bool rin = r < m;
+    T vrow[KW], wrow[KW];   // V(r, k), W(r, k) of the panel's columns done so far (zero beyond)
+#pragma unroll
+    for (int k = 0; k < KW; k++) vrow[k] = wrow[k] = 0;
+#pragma unroll 1
+    for (int jl = 0; jl < KW; jl++) {
+      // 1. column jl minus the panel's earlier reflectors: A(:,jl) -= V W(jl,:)^T + W V(jl,:)^T
+      if (r == jl) {
+#pragma unroll
+        for (int k = 0; k < KW; k++) {
+          rowj[k] = vrow[k];
+          rowj[KW + k] = wrow[k];
+        }
+      }
+      __syncthreads();
+      T c = 0;
+      if (rin && r >= jl) {
+        c = A[(size_t)jl * n + r];
+#pragma unroll
+        for (int k = 0; k < KW; k++) c -= vrow[k] * rowj[KW + k] + wrow[k] * rowj[k];
+      }
+      // 2. reflector annihilating A(jl+2:m, jl) (dlarfg)
+      if (r == jl + 1) sh[0] = c;
+      const T xn2 = block_sum<WGR_MAX>((rin && r >= jl + 2) ? c * c : (T)0, red);
+      const T alpha = sh[0];
+      T beta, tv, scal = 1;
+      if (xn2 == (T)0) {
+        tv = 0;
+        beta = alpha;
+      } else {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tv = (beta - alpha) / beta;
+        scal = (T)1 / (alpha - beta);
+      }
+      T v = 0;
+      if (rin) {
+        if (r == jl) {
+          A[(size_t)jl * n + r] = c;
+          d[jl] = (double)c;
+          e[jl] = (double)beta;
+          tau[jl] = tv;
+        } else if (r == jl + 1) {
+          v = 1;
+          A[(size_t)jl * n + r] = v;
+        } else if (r > jl + 1) {
+          v = c * scal;
+          A[(size_t)jl * n + r] = v;
+        }
+      }
+      vsh[r] = v;
+#pragma unroll
+      for (int k = 0; k < KW; k++) vrow[k] = k == jl ? v : vrow[k];
+      for (int idx = tid; idx < NW_WGR * WGR_MAX; idx += WGR_MAX) (&ypart[0][0])[idx] = 0;
+      __syncthreads();
+      // 3. y = A22 v over the trailing rows [r0, m): lower 64x64 tiles, one wave per tile
+      const int r0 = jl + 1, t = m - r0;
+      const int nseg = (t + 63) / 64, ntri = nseg * (nseg + 1) / 2;
+      const T *S = A + (size_t)r0 * n + r0;   // S(r,c) = S[c*n + r]
+      // every load below reads an element of S(0:t, 0:t) (indices clamped), so stays in the block
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(S, (t - 1) * n * (int)sizeof(T) + t * (int)sizeof(T));
+      for (int tl = wv; tl < ntri; tl += NW_WGR) {
+        int I = (int)((sqrtf(8.0f * tl + 1.0f) - 1.0f) * 0.5f);
+        while ((I + 1) * (I + 2) / 2 <= tl) I++;
+        while (I * (I + 1) / 2 > tl) I--;
+        const int J = tl - I * (I + 1) / 2;
+        const int rr = I * 64 + lane;
+        const bool rok = rr < t;
+        const int voff = min(rr, t - 1) * (int)sizeof(T);
+        const T vI = rok ? vsh[r0 + rr] : (T)0;
+        T yrow = 0;
+#pragma unroll 1
+        for (int qb = 0; qb < 64; qb += 16) {
+          T x[16];
+#pragma unroll
+          for (int q = 0; q < 16; q++) {
+            const int cc = min(J * 64 + qb + q, t - 1);
+            x[q] = buf_load<T>(rs, voff, cc * n * (int)sizeof(T));
+          }
+#pragma unroll
+          for (int q = 0; q < 16; q++) {
+            const int cc = J * 64 + qb + q;
+            const bool in = rok && cc < t && (I > J || rr >= cc);
+            x[q] = in ? x[q] : (T)0;
+            yrow += x[q] * vsh[r0 + cc];   // zero past the trailing rows
+            x[q] = (I == J && rr == cc) ? (T)0 : x[q] * vI;   // the transpose, without the diagonal
+          }
+          const T sz = butterfly16(x);   // lane l: column qb + ((l >> 2) & 15)
+          const int cc = J * 64 + qb + ((lane >> 2) & 15);
+          if ((lane & 3) == 0 && cc < t) ypart[wv][cc] += sz;
+        }
+        if (rok) ypart[wv][rr] += yrow;
+      }
+      __syncthreads();
+      T y = 0;
+      const bool rt = rin && r >= r0;
+      if (rt) {
+#pragma unroll
+        for (int w = 0; w < NW_WGR; w++) y += ypart[w][r - r0];
+      }
+      // 4. W(:,k)^T v and V(:,k)^T v, k < jl, over the trailing rows
+#pragma unroll 1
+      for (int g = 0; g < 2; g++) {
+        T z[KW];
+#pragma unroll
+        for (int k = 0; k < KW; k++) z[k] = (rt && k < jl) ? (g == 0 ? wrow[k] : vrow[k]) * v : (T)0;
+        const T sz = butterfly16(z);
+        if ((lane & 3) == 0) dots[wv][g * KW + ((lane >> 2) & 15)] = sz;
+      }
+      __syncthreads();
+      if (tid < 2 * KW) {
+        T sdot = 0;
+#pragma unroll
+        for (int w = 0; w < NW_WGR; w++) sdot += dots[w][tid];
+        cwv[tid] = sdot;
+      }
+      __syncthreads();
+      // 5. W(:,jl) = tau (y - V (W^T v) - W (V^T v)) - tau/2 (w^T v) v
+      T yy = 0;
+      if (rt && tv != (T)0) {
+        T ay = 0;
+#pragma unroll
+        for (int k = 0; k < KW; k++) ay += vrow[k] * cwv[k] + wrow[k] * cwv[KW + k];
+        yy = tv * (y - ay);
+      }
+      const T a2 = (T)(-0.5) * tv * block_sum<WGR_MAX>(yy * v, red);
+      const T wv_r = rt ? yy + a2 * v : (T)0;
+      if (rt) W[(size_t)jl * n + r] = wv_r;
+#pragma unroll
+      for (int k = 0; k < KW; k++) wrow[k] = k == jl ? wv_r : wrow[k];
+    }
+    // 6. trailing update of the lower triangle: A(KW:m, KW:m) -= V W^T + W V^T, a 64-column tile at a
+    //    time; thread r updates A(r, c) for the tile's columns c <= r
+    const int t = m - KW, nct = (t + 63) / 64;
+    const __amdgpu_buffer_rsrc_t rsA = buf_rsrc(A, (m - 1) * n * (int)sizeof(T) + m * (int)sizeof(T));
+    for (int J = 0; J < nct; J++) {
+      __syncthreads();   // the previous tile's rows are no longer read
+      for (int idx = tid; idx < 64 * KW; idx += WGR_MAX) {
+        const int cl = idx % 64, k = idx / 64, cr = KW + J * 64 + cl;
+        tV[cl * KW + k] = cr < m ? A[(size_t)k * n + cr] : (T)0;   // V(cr, k), cr >= KW > k
+        tW[cl * KW + k] = cr < m ? W[(size_t)k * n + cr] : (T)0;
+      }
+      __syncthreads();
+      if (rin && r >= KW + J * 64) {
+        // columns [0, cmax) of the tile, 16 loads at a time; column indices are clamped to the tile
+        // (uniform), so a lane whose row ends earlier reads elements above the diagonal it leaves alone
+        const int tw = min(64, t - J * 64), cmax = min(tw, r - KW - J * 64 + 1);
+        const int cb = KW + J * 64;
+        for (int c0 = 0; c0 < cmax; c0 += 16) {
+          T x[16];
+#pragma unroll
+          for (int q = 0; q < 16; q++) x[q] = buf_load<T>(rsA, r * (int)sizeof(T), (cb + min(c0 + q, tw - 1)) * n * (int)sizeof(T));
+#pragma unroll
+          for (int q = 0; q < 16; q++) {
+            const int cl = c0 + q;
+#pragma unroll
+            for (int k = 0; k < KW; k++) x[q] -= vrow[k] * tW[cl * KW + k] + wrow[k] * tV[cl * KW + k];
+            if (cl < cmax) A[(size_t)(cb + cl) * n + r] = x[q];
+          }
+        }
+      }
+    }
+    __syncthreads();   // the next panel reads the updated matrix
+  }
 }
 
 #include "syev_sbr.hpp"
@@ -1824,6 +2042,10 @@ struct SyevPlan {
   std::vector<size_t> rc_off, rc_cnt;                  // RCH-wide root chunks
   size_t mrg_max = 0;
   size_t cp_off = 0, cp_cnt = 0, tail_off = 0, tail_cnt = 0;
+  // single-workgroup reduction (k_wg_red): blocks without column-kernel panels, launched beside the
+  // chains, and per group the blocks that continue after their chain
+  size_t wg0_off = 0, wg0_cnt = 0;
+  std::vector<size_t> wgg_off, wgg_cnt;
   std::vector<int> mcap;
   std::vector<GemmDesc> gd;   // products, filled once the arena pointers are known
   // FP64 back-transformation
@@ -1974,6 +2196,23 @@ static int main_panels(int64_t n) {
   return n > TAIL ? (int)((n - TAIL + NB - 1) / NB) : 0;
 }
 
+// Trailing order from which k_wg_red takes over the panels from the column kernels: NRG_WG_RED in
+// the environment (read at every call; at most WGR_MAX). Off (TAIL: every panel to the column
+// kernels) by default: on MI355X one CU's symmetric product of an order-500 trailing matrix takes
+// about 20 us per column (sequential L2 round trips and cross-lane sums), more than the column
+// chain's whole step (about 14 us per column for a lone block, 0.8 MB read by many workgroups), so a
+// lone block of order 512 reduces in 12.6 ms this way against 5.5 ms in the chain.
+static int wg_red_order() {
+  const char *e = std::getenv("NRG_WG_RED");
+  if (!e || !*e) return TAIL;
+  return std::max(TAIL, std::min(WGR_MAX, atoi(e)));
+}
+
+// Panels of a block reduced by the column kernels: those that start with more than `thr` rows.
+static int chain_panels(int64_t n, int thr) {
+  return n > thr ? (int)((n - thr + NB - 1) / NB) : 0;
+}
+
 // Blocks reduced in two stages when NRG_TWO_STAGE=1 in the environment (read at every call): orders
 // above TAIL whose stage-1 panels fit one row per thread. Off by default: on MI355X the bulge chase's
 // critical path (each sweep three dependent steps behind the previous one, about 3n steps of a few
@@ -1990,6 +2229,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
   pl.bd.resize(nblk);
   int64_t zo = 0, vo = 0, wo = 0, ao = 0;
   pl.nsb = 0; pl.P1max = 0; pl.sumAB = 0; pl.sumV2 = 0; pl.sumT2 = 0;
+  const int wthr = wg_red_order();
   for (int b = 0; b < nblk; b++) {
     BlkDesc &d = pl.bd[b];
     d.n = n[b];
@@ -2007,7 +2247,7 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
       pl.sumV2 += (int64_t)SB * n[b] * sb_ks((int)n[b]);
       pl.sumT2 += n[b] * sb_ks((int)n[b]);
     }
-    d.P = d.two ? 0 : main_panels(n[b]);
+    d.P = d.two ? 0 : chain_panels(n[b], wthr);
     d.offYp = pl.sumYp; pl.sumYp += n[b] * ((n[b] + 63) / 64);
     d.offDt = pl.sumYp; pl.sumYp += 64 * ((n[b] + 63) / 64);
     d.offH = off_H[b];
@@ -2123,10 +2363,23 @@ static void build_plan(SyevPlan &pl, int nblk, const int64_t *n, const int64_t *
         pl.cols.push_back(c);
       }
   }
+  // panels left to k_wg_red: from panel P_b to the tail
+  auto wg_left = [&](int b) { return !pl.bd[b].two && main_panels(n[b]) > pl.bd[b].P; };
+  pl.wg0_off = I.size();
+  for (int b : order)
+    if (wg_left(b) && pl.bd[b].P == 0) I.push_back(col_entry(b, 0));
+  pl.wg0_cnt = I.size() - pl.wg0_off;
+  pl.wgg_off.assign(pl.ngrp, 0); pl.wgg_cnt.assign(pl.ngrp, 0);
+  for (int g = 0; g < pl.ngrp; g++) {
+    pl.wgg_off[g] = I.size();
+    for (int b : members[g])
+      if (wg_left(b) && pl.bd[b].P > 0) I.push_back(col_entry(b, pl.bd[b].P));
+    pl.wgg_cnt[g] = I.size() - pl.wgg_off[g];
+  }
   // tail of every block reduced in one stage (one k_tail launch after the column launches)
   pl.tail_off = I.size();
   for (int b : order)
-    if (n[b] >= 1 && !pl.bd[b].two) I.push_back(col_entry(b, pl.bd[b].P));
+    if (n[b] >= 1 && !pl.bd[b].two) I.push_back(col_entry(b, main_panels(n[b])));
   pl.tail_cnt = I.size() - pl.tail_off;
   // two-stage reduction: stage-1 panels slot by slot (block b has panel q in slot q), the band
   // copies, one bulge-chasing workgroup per block, and the stage-2 back-transformation slabs
@@ -2444,11 +2697,15 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
   hipStream_t S[1 + nrg_ctx::NAUX] = {st};
   for (int g = 1; g < NG; g++) S[g] = ctx->aux[g - 1];
   hipStream_t Ssb = ctx->aux[nrg_ctx::NAUX - 1];
-  if (NG > 1 || pl.nsb) {
+  const bool side = pl.nsb || pl.wg0_cnt;   // work on Ssb beside the chains
+  if (NG > 1 || side) {
     NRG_HIP(hipEventRecord(ctx->fork_ev, st));
     for (int g = 1; g < NG; g++) NRG_HIP(hipStreamWaitEvent(S[g], ctx->fork_ev, 0));
-    if (pl.nsb) NRG_HIP(hipStreamWaitEvent(Ssb, ctx->fork_ev, 0));
+    if (side) NRG_HIP(hipStreamWaitEvent(Ssb, ctx->fork_ev, 0));
   }
+  if (pl.wg0_cnt)
+    NRG_TIMED(ctx, Ssb, "k_wg_red", hipLaunchKernelGGL(k_wg_red<T>, grid(pl.wg0_cnt), dim3(WGR_MAX), 0, Ssb, aT,
+                                                       I + pl.wg0_off));
   if (pl.nsb) {
     for (int q = 0; q < pl.P1max; q++) {
       const int *lst = I + pl.sbp_off[q];
@@ -2489,12 +2746,16 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
       launch_gemm<T>(ctx, S[g], L.dGT, I + pl.s2_items[qg], H + pl.s2_items[qg], pl.s2_cnt[qg]);
     }
   }
+  for (int g = 0; g < NG; g++)
+    if (pl.wgg_cnt[g])
+      NRG_TIMED(ctx, S[g], "k_wg_red", hipLaunchKernelGGL(k_wg_red<T>, grid(pl.wgg_cnt[g]), dim3(WGR_MAX), 0, S[g], aT,
+                                                          I + pl.wgg_off[g]));
   if (NG > 1)
     for (int g = 1; g < NG; g++) {
       NRG_HIP(hipEventRecord(ctx->join_ev[g - 1], S[g]));
       NRG_HIP(hipStreamWaitEvent(st, ctx->join_ev[g - 1], 0));
     }
-  if (pl.nsb) {
+  if (side) {
     NRG_HIP(hipEventRecord(ctx->join_ev[nrg_ctx::NAUX - 1], Ssb));
     NRG_HIP(hipStreamWaitEvent(st, ctx->join_ev[nrg_ctx::NAUX - 1], 0));
   }
@@ -2616,6 +2877,7 @@ static std::vector<int64_t> syev_key(nrg_ctx *ctx, int form, int nblk, const int
   std::vector<int64_t> k;
   k.reserve(3 * (size_t)nblk + 16);
   k.push_back(form); k.push_back(nblk); k.push_back(groups); k.push_back(two_stage_on()); k.push_back(setting);
+  k.push_back(wg_red_order());
   for (const void *q : {(const void *)H, (const void *)w, (const void *)st, (const void *)ctx->ws.p,
                         (const void *)ctx->ws2.p, (const void *)ctx->dcbuf.p, (const void *)ctx->flags.p})
     k.push_back((int64_t)(intptr_t)q);

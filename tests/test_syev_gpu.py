@@ -205,6 +205,20 @@ def test_step_batch_256_blocks(ctx, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("handover", ["512", "300", "128"])
+def test_single_workgroup_reduction(ctx, mode, handover, monkeypatch):
+    """Orders around the boundaries of the single-workgroup reduction (k_wg_red: 16-column panels
+    from the trailing order given by NRG_WG_RED, at most 512, down to the tail order 128) and of the
+    column kernels that hand the block over to it; 128 leaves every panel to the column kernels."""
+    monkeypatch.setenv("NRG_WG_RED", handover)
+    rng = np.random.default_rng(17)
+    orders = [129, 144, 145, 160, 161, 200, 255, 256, 299, 300, 301, 383, 385, 511, 512, 513, 545, 600, 700]
+    blocks = [workloads.eig_block(rng, n)[0] for n in orders]
+    blocks.append(np.triu(np.full((300, 300), 1.0)))   # rank one: reflectors with tau = 0
+    check_against_oracle(ctx, blocks, mode, label=f"handover {handover}")
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_large_orders(ctx, mode):
     rng = np.random.default_rng(15)
     # more than 1024 trailing rows: several rows per thread in the column preparation
