@@ -492,11 +492,13 @@ def bench_fdm(args):
     ctx.kernel_timing(False)
     log("  " + ", ".join(f"{k} {v[0] / 2:.3f} ms ({v[1] // 2} launches)" for k, v in kt.items()))
     abytes = 16.0 * nel + 8.0 * sum(p["dim_i"] + p["dim_j"] for p in pairs)
+    # kernel time per step (over the two steps timed)
     tms, cnt = kt["k_fdm_bin"]
-    ach = abytes / (tms / cnt * 1e-3) / 1e9
+    ach = abytes / (tms / 2 * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": "k_fdm_bin", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "note": "algorithmic bytes = 16 B per element (op1 + op2 or the rho product) + 8 B per energy"}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "launches_per_step": cnt // 2,
+            "note": "algorithmic bytes = 16 B per element (op1 + op2 or the rho product) + 8 B per energy, over "
+                    "the step's k_fdm_bin time"}
     cpu = None
     if not args.no_cpu:
         sample = pairs

@@ -35,18 +35,6 @@ struct FdmArgs {
   double *gt_pos, *gt_neg, *ls_pos, *ls_neg;
 };
 
-// Bins::add + Bins::add_std (c++/bins.hpp:103-132) with the energy already made non-negative
-__device__ __forceinline__ void bin_add(double discard, double *bins, double energy, double w, int64_t idx, double rem,
-                                        bool low) {
-  if (w == 0.0 || fabs(w) < discard * energy) return;
-  if (low) {
-    atomicAdd(bins + idx, w);
-    return;
-  }
-  atomicAdd(bins + idx, (1.0 - rem) * w);
-  atomicAdd(bins + idx + 1, rem * w);
-}
-
 // Bin position of a non-negative energy (Bins::add_std): below zero_epsilon = 1e-14 (c++/bins.hpp:46)
 // or below the grid -> bin 0, above it -> the last bin (low = true: the whole weight in bin idx);
 // otherwise split between idx and idx + 1 with weights 1 - rem and rem.
@@ -66,6 +54,87 @@ __device__ __forceinline__ void bin_pos(double energy, double log10emin, double 
   idx = (int64_t)ip;
   rem = x - ip;
   low = false;
+}
+
+// A tile's window of the histograms in LDS. Along a row of a tile the energy difference E_j - E_i is
+// monotone and the tile's differences span a narrow range of bins, except near zero: for each sign s
+// of E_j - E_i (0: >= 0) and each of NH histograms, the window holds the WINB bins below the tile's
+// largest |E_j - E_i| of that sign (one decade at 1000 bins per decade). An element whose bins fall
+// inside adds there with an LDS atomic; the others add to the replica in global memory; the window is
+// then added into the replica, one global atomic per non-zero bin.
+constexpr int WINB = 1024;
+
+template <int NH>
+struct BinWindow {
+  double *lds;     // [2][NH][WINB]
+  int64_t lo[2];   // first bin of each sign's window
+  __device__ double *at(int s, int h) const { return lds + (s * NH + h) * WINB; }
+};
+
+// lo of a sign's window from the largest difference dtop of that sign (none when dtop < 0)
+__device__ __forceinline__ int64_t window_lo(double dtop, double log10emin, double bpd, int64_t nbins) {
+  int64_t idx;
+  double rem;
+  bool low;
+  bin_pos(dtop > 0.0 ? dtop : 0.0, log10emin, bpd, nbins, idx, rem, low);
+  int64_t lo = idx + 2 - WINB;
+  lo = lo < nbins - WINB ? lo : nbins - WINB;
+  return lo > 0 ? lo : 0;
+}
+
+// Bins::add + Bins::add_std (c++/bins.hpp:103-132) with the energy already made non-negative: into the
+// window when both bins of the element are inside it, else into the replica
+__device__ __forceinline__ void win_add(double discard, double *win, int64_t lo, double *bins, double energy, double w,
+                                        int64_t idx, double rem, bool low) {
+  if (w == 0.0 || fabs(w) < discard * energy) return;
+  const int64_t b = idx - lo;
+  if (low) {
+    if (b >= 0 && b < WINB) atomicAdd(win + b, w);
+    else atomicAdd(bins + idx, w);
+    return;
+  }
+  if (b >= 0 && b + 1 < WINB) {
+    atomicAdd(win + b, (1.0 - rem) * w);
+    atomicAdd(win + b + 1, rem * w);
+  } else {
+    atomicAdd(bins + idx, (1.0 - rem) * w);
+    atomicAdd(bins + idx + 1, rem * w);
+  }
+}
+
+// Window placement: the tile's energies are in LDS (e_i[0, ni), e_j[0, nj)); wave 0 reduces e_i, wave 1
+// e_j, and every thread then reads the two windows' first bins. Also clears the window.
+template <int NH>
+__device__ __forceinline__ void window_setup(BinWindow<NH> &w, double *lds, const double *e_i, int ni, const double *e_j,
+                                             int nj, double log10emin, double bpd, int64_t nbins, double *mm) {
+  w.lds = lds;
+  for (int k = threadIdx.x; k < 2 * NH * WINB; k += 256) lds[k] = 0.0;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (wv < 2) {
+    const double *e = wv == 0 ? e_i : e_j;
+    const bool in = l < (wv == 0 ? ni : nj);
+    const double hi = wave_max(in ? e[l] : -INFINITY), lo = -wave_max(in ? -e[l] : -INFINITY);
+    if (l == 0) {
+      mm[2 * wv] = lo;
+      mm[2 * wv + 1] = hi;
+    }
+  }
+  __syncthreads();
+  // mm: min E_i, max E_i, min E_j, max E_j
+  w.lo[0] = window_lo(mm[3] - mm[0], log10emin, bpd, nbins);
+  w.lo[1] = window_lo(mm[1] - mm[2], log10emin, bpd, nbins);
+}
+
+// adds the window into the replicas (hist(s, h) = the replica of histogram h for sign s)
+template <int NH, typename F>
+__device__ __forceinline__ void window_flush(const BinWindow<NH> &w, int64_t nbins, F hist) {
+  __syncthreads();
+  for (int k = threadIdx.x; k < 2 * NH * WINB; k += 256) {
+    const double v = w.lds[k];
+    const int s = k / (NH * WINB), h = (k / WINB) % NH;
+    const int64_t b = w.lo[s] + k % WINB;
+    if (v != 0.0 && b < nbins) atomicAdd(hist(s, h) + b, v);
+  }
 }
 
 // The scatter goes to FDM_COPIES replicas of the histograms; k_fdm_reduce then adds the replicas
@@ -222,12 +291,15 @@ __global__ __launch_bounds__(256) void k_fdm_bin(FdmArgs a, const int2 *items) {
     Ej[threadIdx.x - 64] = e;
     Bj[threadIdx.x - 64] = exp(-e / a.T);
   }
+  __shared__ double wlds[2 * 2 * WINB], mm[4];
   __syncthreads();
+  BinWindow<2> win;
+  window_setup(win, wlds, Ei, (int)min<int64_t>(64, P.dim_i - i0), Ej, (int)min<int64_t>(64, P.dim_j - j0), a.log10emin,
+               a.bpd, a.nbins, mm);
   const int il = threadIdx.x & 63;
   const int i = i0 + il;
-  if (i >= P.dim_i) return;
   const bool Di = i >= P.K_i;
-  for (int jl = threadIdx.x >> 6; jl < 64; jl += 4) {
+  for (int jl = threadIdx.x >> 6; jl < 64 && i < P.dim_i; jl += 4) {
     const int j = j0 + jl;
     if (j >= P.dim_j) break;
     const bool Dj = j >= P.K_j;
@@ -249,9 +321,13 @@ __global__ __launch_bounds__(256) void k_fdm_bin(FdmArgs a, const int2 *items) {
     double rem;
     bool low;
     bin_pos(energy, a.log10emin, a.bpd, a.nbins, idx, rem, low);
-    bin_add(a.discard, pos ? a.gt_pos : a.gt_neg, energy, wgt, idx, rem, low);
-    bin_add(a.discard, pos ? a.ls_pos : a.ls_neg, energy, wls, idx, rem, low);
+    const int sg = pos ? 0 : 1;
+    win_add(a.discard, win.at(sg, 0), win.lo[sg], pos ? a.gt_pos : a.gt_neg, energy, wgt, idx, rem, low);
+    win_add(a.discard, win.at(sg, 1), win.lo[sg], pos ? a.ls_pos : a.ls_neg, energy, wls, idx, rem, low);
   }
+  window_flush(win, a.nbins, [&](int sg, int h) {
+    return h == 0 ? (sg == 0 ? a.gt_pos : a.gt_neg) : (sg == 0 ? a.ls_pos : a.ls_neg);
+  });
 }
 
 int fdm_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double T, double wnf,
@@ -328,11 +404,14 @@ __global__ __launch_bounds__(256) void k_dmnrg_bin(DmnrgArgs a, const int2 *item
     const int j = j0 + threadIdx.x - 64;
     Ej[threadIdx.x - 64] = j < P.K_j ? P.E_j[j] : 0.0;
   }
+  __shared__ double wlds[2 * WINB], mm[4];
   __syncthreads();
+  BinWindow<1> win;
+  window_setup(win, wlds, Em, (int)min<int64_t>(64, P.K_i - m0), Ej, (int)min<int64_t>(64, P.K_j - j0), a.log10emin,
+               a.bpd, a.nbins, mm);
   const int ml = threadIdx.x & 63;
   const int m = m0 + ml;
-  if (m >= P.K_i) return;
-  for (int jl = threadIdx.x >> 6; jl < 64; jl += 4) {
+  for (int jl = threadIdx.x >> 6; jl < 64 && m < P.K_i; jl += 4) {
     const int j = j0 + jl;
     if (j >= P.K_j) break;
     const double de = Ej[jl] - Em[ml];
@@ -346,8 +425,10 @@ __global__ __launch_bounds__(256) void k_dmnrg_bin(DmnrgArgs a, const int2 *item
     double rem;
     bool low;
     bin_pos(energy, a.log10emin, a.bpd, a.nbins, idx, rem, low);
-    bin_add(a.discard, de >= 0.0 ? pos : neg, energy, w, idx, rem, low);
+    const int sg = de >= 0.0 ? 0 : 1;
+    win_add(a.discard, win.at(sg, 0), win.lo[sg], sg == 0 ? pos : neg, energy, w, idx, rem, low);
   }
+  window_flush(win, a.nbins, [&](int sg, int) { return sg == 0 ? pos : neg; });
 }
 
 int dmnrg_run(nrg_ctx *ctx, const nrg_fdm_pair *pairs, int npairs, const nrg_bin_grid *grid, double emin, double emax,
@@ -428,12 +509,15 @@ __global__ __launch_bounds__(256) void k_cfs_bin(CfsArgs a, const int2 *items) {
     Ej[threadIdx.x - 64] = e;
     Bj[threadIdx.x - 64] = a.last ? exp(-e / a.T) : 0.0;
   }
+  __shared__ double wlds[2 * 2 * WINB], mm[4];
   __syncthreads();
+  BinWindow<2> win;
+  window_setup(win, wlds, Ei, (int)min<int64_t>(64, P.dim_i - i0), Ej, (int)min<int64_t>(64, P.dim_j - j0), a.log10emin,
+               a.bpd, a.nbins, mm);
   const int il = threadIdx.x & 63;
   const int i = i0 + il;
-  if (i >= P.dim_i) return;
   const bool ki = i < P.K_i;
-  for (int jl = threadIdx.x >> 6; jl < 64; jl += 4) {
+  for (int jl = threadIdx.x >> 6; jl < 64 && i < P.dim_i; jl += 4) {
     const int j = j0 + jl;
     if (j >= P.dim_j) break;
     const bool kj = j < P.K_j;
@@ -460,9 +544,11 @@ __global__ __launch_bounds__(256) void k_cfs_bin(CfsArgs a, const int2 *items) {
     double rem;
     bool low;
     bin_pos(energy, a.log10emin, a.bpd, a.nbins, idx, rem, low);
-    bin_add(a.discard, de >= 0.0 ? gp : gn, energy, wg * P.factor, idx, rem, low);
-    bin_add(a.discard, de >= 0.0 ? lp : ln, energy, wl * P.factor, idx, rem, low);
+    const int sg = de >= 0.0 ? 0 : 1;
+    win_add(a.discard, win.at(sg, 0), win.lo[sg], sg == 0 ? gp : gn, energy, wg * P.factor, idx, rem, low);
+    win_add(a.discard, win.at(sg, 1), win.lo[sg], sg == 0 ? lp : ln, energy, wl * P.factor, idx, rem, low);
   }
+  window_flush(win, a.nbins, [&](int sg, int h) { return h == 0 ? (sg == 0 ? gp : gn) : (sg == 0 ? lp : ln); });
 }
 
 // last: 0 / 1 as step.last() for Algo_CFS; 2 for Algo_FT (gt_pos / gt_neg receive the weights, the
