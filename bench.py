@@ -316,7 +316,7 @@ def autotune_summary(table, calls):
     return out
 
 
-BF16_PEAK_TFLOPS = 2500.0
+INT8_PEAK_TOPS = 5000.0
 
 
 def timed_calls(step, steps, warmup, sync):
@@ -407,10 +407,10 @@ def bench_recalc(args):
     if mode == "fp64":
         roof.update(peak=FP64_PEAK_TFLOPS, frac=round(ach / FP64_PEAK_TFLOPS, 4))
     else:
-        slice_tf = ach * 28   # 28 BF16 slice products per FP64 product
-        roof.update(peak=BF16_PEAK_TFLOPS, frac=round(slice_tf / BF16_PEAK_TFLOPS, 4),
-                    note="achieved is FP64-equivalent; frac is the BF16 slice-product rate (28 slice "
-                         "products per FP64 product) over the 2.5 PFLOP/s dense BF16 peak")
+        slice_tops = ach * 36   # 36 INT8 slice products per FP64 product (8 slices, p + q < 8)
+        roof.update(peak=INT8_PEAK_TOPS, unit_peak="TOP/s", frac=round(slice_tops / INT8_PEAK_TOPS, 4),
+                    note="achieved is FP64-equivalent; frac is the INT8 slice-product rate (36 slice "
+                         "products per FP64 product) over the 5.0 POP/s dense INT8 peak")
     cpu = None
     if not args.no_cpu:
         # oracle dgemm on the host cores over a bounded sample of the table
@@ -449,7 +449,7 @@ def bench_recalc(args):
     out = {"metric": "batched U^T O U recalc GFLOP/s (FP64-equivalent), split-precision vs FP64 at 1e-12 normwise",
            "value": round(flops / ms / 1e6, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "bf16-slices+f32+f64" if mode == "ozaki" else "f64", "data": "synthetic",
+           "vs_baseline": None, "dtype": "int8-slices+i32+f64" if mode == "ozaki" else "f64", "data": "synthetic",
            "config": {"workload": f"recalc_general, {len(table)} outputs, dims log-uniform [16,2000], 16-row QSZ "
                                   "2ch doublet tables, Haar U, N(0,1) O, seed 20261205", "form": mode,
                       "flops_per_step": flops},

@@ -38,7 +38,7 @@ extern "C" {
 
 /* product precisions */
 #define NRG_GEMM_FP64     0   /* FP64 arithmetic */
-#define NRG_GEMM_OZAKI    1   /* split-precision BF16 slices, FP32 accumulation, FP64 recombination */
+#define NRG_GEMM_OZAKI    1   /* split-precision INT8 slices, exact INT32 accumulation, FP64 recombination */
 
 typedef struct nrg_ctx nrg_ctx;
 

@@ -2,17 +2,17 @@
 //
 // Same table semantics and evaluation order as the FP64 form (recalc_f / recalc_general / cdmI,
 // c++/recalc.hpp:52-108, c++/dmnrg.hpp:54-80, c++/numerics_ublas.hpp:108-140); each FP64 product is
-// evaluated with the Ozaki error-free splitting on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16):
+// evaluated with the Ozaki error-free splitting on the INT8 matrix cores (v_mfma_i32_16x16x64_i8):
 //
 //   1. k_oz_split: every operand row x (a row of op(A), or a column of op(B)) gets an exponent E with
-//      |x_k| < 2^E, and OZ_S integer slices m_p(k) with |m_p| < 2^OZ_BITS,
+//      |x_k| < 2^E, and OZ_S signed integer slices m_p(k) with |m_p| < 2^OZ_BITS (int8),
 //          x_k = sum_p m_p(k) 2^(E - OZ_BITS (p + 1)) + remainder,  |remainder| < 2^(E - OZ_S OZ_BITS).
-//      Integers below 2^8 are exact in BF16.
-//   2. k_oz_gemm: for every pair of slices with p + q < OZ_S (28 pairs) the slice products are summed
-//      on the matrix cores in FP32. A 32-long dot product of slice integers is below 2^21 in
-//      magnitude, and at most OZ_S such products share a diagonal d = p + q, so each diagonal's FP32
-//      sum over one 32-wide K step is exact (< 2^24). After every K step the diagonals are scaled by
-//      2^(-8 d) and by the row/column exponents and added into FP64 accumulators.
+//   2. k_oz_gemm: for every pair of slices with p + q < OZ_S (36 pairs) the slice products are summed
+//      on the matrix cores in INT32, exactly: a product of two slices is at most 127^2 < 2^14 in
+//      magnitude and at most OZ_S pairs share a diagonal d = p + q, so a diagonal's sum over OZ_KFLUSH
+//      K values stays below 2^31. Each diagonal is converted once per OZ_KFLUSH of K (once for the
+//      shapes of a recalc table), scaled by 2^(-7 d) and by the row/column exponents, and added into
+//      FP64 accumulators.
 // Dropped terms (p + q >= OZ_S, remainders) are below K 2^(E_row + F_col) 10 2^-56, i.e. a normwise
 // relative error of order 1e-15 for the shapes of a recalc table.
 #include <cstring>
@@ -23,25 +23,26 @@
 
 namespace nrg {
 
-constexpr int OZ_S = 7;      // slices per operand
-constexpr int OZ_BITS = 8;   // bits per slice
-constexpr int OZ_KC = 32;    // K of one BF16 matrix-core step
-constexpr int OZ_LDK = 40;   // LDS row stride (bf16): 32 + 8 to spread the banks
+constexpr int OZ_S = 8;        // slices per operand
+constexpr int OZ_BITS = 7;     // bits per slice (magnitude; int8)
+constexpr int OZ_KC = 64;      // K of one INT8 matrix-core step
+constexpr int OZ_LDK = 80;     // LDS row stride (bytes): 64 + 16 to spread the banks
+constexpr int OZ_KFLUSH = 16384;   // K values summed in INT32 before a conversion: 8 * 2^14 * 16384 = 2^31
+static_assert(OZ_KFLUSH % OZ_KC == 0, "conversions fall on K steps");
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));   // 16 int8 (an MFMA operand) or 4 int32
 
 // Operand rows to split: row r of an R x K matrix Y whose element (r, k) is
 // t ? X[r * ld + k] : X[k * ld + r] (column-major storage conventions, as GemmPart).
 struct OzSplit {
   const double *X;
   int ld, t, R, K, Kp;   // Kp = K rounded up to OZ_KC; slices are zero beyond K
-  __bf16 *S;             // slice p of row r at S + ((size_t)p * R + r) * Kp
+  int8_t *S;             // slice p of row r at S + ((size_t)p * R + r) * Kp
   int *E;                // per-row exponent
 };
 
 struct OzPart {          // scale * op(A) op(B): A rows = slices of op(A) rows, B rows = slices of op(B) columns
-  const __bf16 *SA, *SB;
+  const int8_t *SA, *SB;
   const int *EA, *EB;
   int MA, NB, Kp;
   double scale;
@@ -54,38 +55,100 @@ struct OzDesc {          // column-major C (M x N) = alpha * sum of parts + beta
   int p0, np;
 };
 
+// One workgroup per 64 rows of an operand. Reads run along the operand's storage (lanes along r when
+// rows are strided, along k when they are contiguous); the slices of a 64 x 64 (row, k) block are
+// formed in LDS and written out along k, 16 bytes per thread.
+constexpr int OZ_SR = 64;   // rows per split workgroup
 __global__ __launch_bounds__(256) void k_oz_split(const OzSplit *sp, const int2 *items) {
   const int2 it = items[blockIdx.x];
   const OzSplit S = sp[it.x];
-  const int lane = threadIdx.x & 63, r = it.y * 4 + (threadIdx.x >> 6);
-  if (r >= S.R) return;   // whole waves leave together; no barrier follows
-  double mx = 0.0;
-  for (int k = lane; k < S.K; k += 64) mx = fmax(mx, fabs(S.t ? S.X[(size_t)r * S.ld + k] : S.X[(size_t)k * S.ld + r]));
-  mx = wave_max(mx);
-  int E = 0;
-  if (mx > 0.0) frexp(mx, &E);   // mx < 2^E
-  if (lane == 0) S.E[r] = E;
-  for (int k = lane; k < S.Kp; k += 64) {
-    double x = k < S.K ? (S.t ? S.X[(size_t)r * S.ld + k] : S.X[(size_t)k * S.ld + r]) : 0.0;
+  const int r0 = it.y * OZ_SR, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ double red[4][OZ_SR];
+  __shared__ int Er[OZ_SR];
+  __shared__ __attribute__((aligned(16))) int8_t T[OZ_S][OZ_SR][64 + 16];
+  // element (r0 + a, k) with a < 64: strided rows are read with lanes along a, contiguous rows along k
+  auto elem = [&](int a, int k) -> double {
+    const int r = r0 + a;
+    if (r >= S.R || k >= S.K) return 0.0;
+    return S.t ? S.X[(size_t)r * S.ld + k] : S.X[(size_t)k * S.ld + r];
+  };
+  // 1. row exponents: |x| < 2^E over the row
+  {
+    if (!S.t) {   // lane = row; wave w reads k = 16 w + 64 i + [0, 16): 16 loads in flight per thread
+      double mx = 0.0;
+      for (int kb = 16 * wv; kb < S.K; kb += 64) {
+        double x[16];
 #pragma unroll
-    for (int p = 0; p < OZ_S; p++) {
-      const int sh = E - OZ_BITS * (p + 1);
-      const double q = trunc(ldexp(x, -sh));   // |q| < 2^OZ_BITS, exact
-      x -= ldexp(q, sh);                       // exact remainder
-      S.S[((size_t)p * S.R + r) * S.Kp + k] = (__bf16)(float)q;
+        for (int q = 0; q < 16; q++) x[q] = elem(lane, kb + q);
+#pragma unroll
+        for (int q = 0; q < 16; q++) mx = fmax(mx, fabs(x[q]));
+      }
+      red[wv][lane] = mx;
+    } else {      // wave w: rows w, w + 4, ...; lanes along k
+      for (int a = wv; a < OZ_SR; a += 4) {
+        double m = 0.0;
+        for (int kb = 0; kb < S.K; kb += 64 * 8) {
+          double x[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++) x[q] = elem(a, kb + 64 * q + lane);
+#pragma unroll
+          for (int q = 0; q < 8; q++) m = fmax(m, fabs(x[q]));
+        }
+        m = wave_max(m);
+        if (lane == 0) red[0][a] = m;
+      }
+      if (lane < OZ_SR) red[1][lane] = red[2][lane] = red[3][lane] = 0.0;   // only red[0] carries values
     }
+    __syncthreads();
+    if (tid < OZ_SR) {
+      const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+      int E = 0;
+      if (m > 0.0) frexp(m, &E);   // m < 2^E
+      Er[tid] = E;
+      if (r0 + tid < S.R) S.E[r0 + tid] = E;
+    }
+    __syncthreads();
+  }
+  // 2. slices, one 64-wide K block at a time (Kp is a multiple of 64; zero beyond K)
+  for (int k0 = 0; k0 < S.Kp; k0 += 64) {
+    // element (a, kl) of the block: strided rows a = lane, kl = 16 wv + q; contiguous rows a = 16 wv + q,
+    // kl = lane; the 16 loads go out together
+    double xs[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) xs[q] = elem(S.t ? 16 * wv + q : lane, k0 + (S.t ? lane : 16 * wv + q));
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int a = S.t ? 16 * wv + q : lane, kl = S.t ? lane : 16 * wv + q;
+      double x = xs[q];
+      const int E = Er[a];
+#pragma unroll
+      for (int p = 0; p < OZ_S; p++) {
+        const int sh = E - OZ_BITS * (p + 1);
+        const double m = trunc(ldexp(x, -sh));   // |m| < 2^OZ_BITS, exact
+        x -= ldexp(m, sh);                       // exact remainder
+        T[p][a][kl] = (int8_t)m;
+      }
+    }
+    __syncthreads();
+    // write out: slice p, row a, 16 bytes at 16 c; 8 x 64 x 4 chunks over 256 threads
+    for (int v = tid; v < OZ_S * OZ_SR * 4; v += 256) {
+      const int p = v / (OZ_SR * 4), a = (v / 4) % OZ_SR, c = v % 4;
+      if (r0 + a < S.R)
+        *(i32x4 *)(S.S + ((size_t)p * S.R + r0 + a) * S.Kp + k0 + 16 * c) = *(const i32x4 *)&T[p][a][16 * c];
+    }
+    __syncthreads();
   }
 }
 
-// two waves per SIMD (at most 256 registers per lane; the unconstrained allocation takes 264 and
-// leaves one): 1.33x on the recalc table
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_oz_gemm(const OzDesc *descs, const OzPart *parts, const int2 *items) {
+// one wave per SIMD: the INT32 accumulators (128 registers) and the next K step's slices in flight
+// take more than the 256 registers per lane of two waves
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_oz_gemm(const OzDesc *descs, const OzPart *parts, const int2 *items) {
   const int2 it = items[blockIdx.x];
   const OzDesc G = descs[it.x];
   const int tm = (G.M + 63) / 64;
   const int m0 = (it.y % tm) * 64, n0 = (it.y / tm) * 64;
-  __shared__ __attribute__((aligned(16))) __bf16 As[OZ_S][64][OZ_LDK];   // slices of op(A) rows m0..m0+63
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[OZ_S][64][OZ_LDK];   // slices of op(B) columns n0..n0+63
+  __shared__ __attribute__((aligned(16))) int8_t As[OZ_S][64][OZ_LDK];   // slices of op(A) rows m0..m0+63
+  __shared__ __attribute__((aligned(16))) int8_t Bs[OZ_S][64][OZ_LDK];   // slices of op(B) columns n0..n0+63
   __shared__ int Ea[64], Eb[64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
@@ -111,53 +174,72 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int r = 0; r < 4; r++)
           sh[i][j][r] = Ea[wm + 16 * i + (lane & 15)] + Eb[wn + 16 * j + 4 * (lane >> 4) + r] - 2 * OZ_BITS;
-    for (int k0 = 0; k0 < P.Kp; k0 += OZ_KC) {
-      for (int v = tid; v < OZ_S * 256; v += 256) {
-        const int p = v >> 8, row = (v >> 2) & 63, kv = v & 3;
-        bf16x8 a = {}, b = {};
-        if (m0 + row < P.MA) a = *(const bf16x8 *)(P.SA + ((size_t)p * P.MA + m0 + row) * P.Kp + k0 + 8 * kv);
-        if (n0 + row < P.NB) b = *(const bf16x8 *)(P.SB + ((size_t)p * P.NB + n0 + row) * P.Kp + k0 + 8 * kv);
-        *(bf16x8 *)&As[p][row][8 * kv] = a;
-        *(bf16x8 *)&Bs[p][row][8 * kv] = b;
-      }
-      __syncthreads();
-      f32x4 acc[OZ_S][2][2];
-#pragma unroll
-      for (int d = 0; d < OZ_S; d++)
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++) acc[d][i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-      bf16x8 fb[OZ_S][2];   // MFMA A operand: op(B) columns (rows of D)
-#pragma unroll
-      for (int q = 0; q < OZ_S; q++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) fb[q][j] = *(const bf16x8 *)&Bs[q][wn + 16 * j + (lane & 15)][8 * (lane >> 4)];
+    i32x4 acc[OZ_S][2][2];   // exact INT32 sums per diagonal p + q
+    // the slices of one K step, 16 bytes of each of OZ_S rows of op(A) and of op(B) per thread: the
+    // next step's are loaded while the matrix cores work on this one
+    i32x4 ra[OZ_S], rb[OZ_S];
+    const int lrow = (tid >> 2) & 63, lkv = tid & 3;
+    auto load_step = [&](int k0) {
 #pragma unroll
       for (int p = 0; p < OZ_S; p++) {
-        bf16x8 fa[2];   // MFMA B operand: op(A) rows (columns of D)
+        ra[p] = (i32x4){0, 0, 0, 0};
+        rb[p] = (i32x4){0, 0, 0, 0};
+        if (m0 + lrow < P.MA) ra[p] = *(const i32x4 *)(P.SA + ((size_t)p * P.MA + m0 + lrow) * P.Kp + k0 + 16 * lkv);
+        if (n0 + lrow < P.NB) rb[p] = *(const i32x4 *)(P.SB + ((size_t)p * P.NB + n0 + lrow) * P.Kp + k0 + 16 * lkv);
+      }
+    };
+    load_step(0);
+    for (int k0 = 0; k0 < P.Kp; k0 += OZ_KC) {
+      if (k0 % OZ_KFLUSH == 0) {
 #pragma unroll
-        for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8 *)&As[p][wm + 16 * i + (lane & 15)][8 * (lane >> 4)];
+        for (int d = 0; d < OZ_S; d++)
 #pragma unroll
-        for (int q = 0; q < OZ_S - p; q++)
+          for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[d][i][j] = (i32x4){0, 0, 0, 0};
+      }
+      // 256 threads x 16 bytes cover the 64 rows x 64 bytes of a slice: thread t, row (t / 4) % 64,
+      // bytes 16 (t % 4), slice p
+#pragma unroll
+      for (int p = 0; p < OZ_S; p++) {
+        *(i32x4 *)&As[p][lrow][16 * lkv] = ra[p];
+        *(i32x4 *)&Bs[p][lrow][16 * lkv] = rb[p];
+      }
+      __syncthreads();
+      if (k0 + OZ_KC < P.Kp) load_step(k0 + OZ_KC);
+      // MFMA operands: lane l holds 16 K values (16 (l / 16) .. + 16) of row l % 16; A operand: op(B)
+      // columns (rows of D), B operand: op(A) rows (columns of D)
+#pragma unroll
+      for (int p = 0; p < OZ_S; p++) {
+        i32x4 fa[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) fa[i] = *(const i32x4 *)&As[p][wm + 16 * i + (lane & 15)][16 * (lane >> 4)];
+#pragma unroll
+        for (int q = 0; q < OZ_S - p; q++) {
+          i32x4 fb[2];
+#pragma unroll
+          for (int j = 0; j < 2; j++) fb[j] = *(const i32x4 *)&Bs[q][wn + 16 * j + (lane & 15)][16 * (lane >> 4)];
 #pragma unroll
           for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int j = 0; j < 2; j++)
-              acc[p + q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[q][j], fa[i], acc[p + q][i][j], 0, 0, 0);
+              acc[p + q][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fb[j], fa[i], acc[p + q][i][j], 0, 0, 0);
+        }
       }
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            double t = 0.0;
-#pragma unroll
-            for (int d = OZ_S - 1; d >= 0; d--) t += ldexp((double)acc[d][i][j][r], -OZ_BITS * d);
-            accC[i][j][r] = fma(P.scale, ldexp(t, sh[i][j][r]), accC[i][j][r]);
-          }
       __syncthreads();
+      if ((k0 + OZ_KC) % OZ_KFLUSH == 0 || k0 + OZ_KC >= P.Kp) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              double t = 0.0;
+#pragma unroll
+              for (int d = OZ_S - 1; d >= 0; d--) t += ldexp((double)acc[d][i][j][r], -OZ_BITS * d);
+              accC[i][j][r] = fma(P.scale, ldexp(t, sh[i][j][r]), accC[i][j][r]);
+            }
+      }
     }
   }
 #pragma unroll
@@ -191,8 +273,8 @@ struct OzBuild {
     OzSplit s{};
     s.X = X; s.ld = ld; s.t = t; s.R = R; s.K = K;
     s.Kp = (K + OZ_KC - 1) / OZ_KC * OZ_KC;
-    s.S = (__bf16 *)(uintptr_t)ws_bytes;   // offset for now, rebased later
-    ws_bytes += ((size_t)OZ_S * R * s.Kp * sizeof(__bf16) + 255) & ~(size_t)255;
+    s.S = (int8_t *)(uintptr_t)ws_bytes;   // offset for now, rebased later
+    ws_bytes += ((size_t)OZ_S * R * s.Kp + 255) & ~(size_t)255;
     s.E = (int *)(uintptr_t)ws_bytes;
     ws_bytes += ((size_t)R * sizeof(int) + 255) & ~(size_t)255;
     splits.push_back(s);
@@ -201,8 +283,8 @@ struct OzBuild {
   // one part: scale * op(A) op(B) with op(A) M x K (flags of GemmPart) and op(B) K x N
   void part(const double *A, int lda, int tA, const double *B, int ldb, int tB, int M, int N, int K, double scale) {
     OzPart p{};
-    p.SA = (const __bf16 *)(uintptr_t)split(A, lda, tA, M, K);   // split indices for now
-    p.SB = (const __bf16 *)(uintptr_t)split(B, ldb, !tB, N, K);
+    p.SA = (const int8_t *)(uintptr_t)split(A, lda, tA, M, K);   // split indices for now
+    p.SB = (const int8_t *)(uintptr_t)split(B, ldb, !tB, N, K);
     p.MA = M; p.NB = N; p.Kp = (K + OZ_KC - 1) / OZ_KC * OZ_KC; p.scale = scale;
     parts.push_back(p);
   }
@@ -243,7 +325,7 @@ int gemm3_ozaki(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream
   if (int r = ctx->ws.ensure(b.ws_bytes + 256)) return r;
   char *wsb = ctx->ws.as<char>();
   for (OzSplit &s : b.splits) {
-    s.S = (__bf16 *)(wsb + (uintptr_t)s.S);
+    s.S = (int8_t *)(wsb + (uintptr_t)s.S);
     s.E = (int *)(wsb + (uintptr_t)s.E);
   }
   for (OzPart &p : b.parts) {
@@ -251,11 +333,11 @@ int gemm3_ozaki(nrg_ctx *ctx, const nrg_gemm3_term *terms, int nterms, hipStream
     p.SA = sa.S; p.EA = sa.E;
     p.SB = sb.S; p.EB = sb.E;
   }
-  // work items: split (split, group of 4 rows); product (descriptor, 64 x 64 tile)
+  // work items: split (split, group of OZ_SR rows); product (descriptor, 64 x 64 tile)
   std::vector<int> si[2], gi[2];
   for (int stage = 0; stage < 2; stage++) {
     for (size_t s = stage ? split_end[0] : 0; s < split_end[stage]; s++)
-      for (int g = 0; g < (b.splits[s].R + 3) / 4; g++) { si[stage].push_back((int)s); si[stage].push_back(g); }
+      for (int g = 0; g < (b.splits[s].R + OZ_SR - 1) / OZ_SR; g++) { si[stage].push_back((int)s); si[stage].push_back(g); }
     for (size_t d = stage ? desc_end[0] : 0; d < desc_end[stage]; d++) {
       const int t = ((b.descs[d].M + 63) / 64) * ((b.descs[d].N + 63) / 64);
       for (int k = 0; k < t; k++) { gi[stage].push_back((int)d); gi[stage].push_back(k); }

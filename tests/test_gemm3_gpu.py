@@ -97,7 +97,7 @@ def test_bad_terms_are_rejected(ctx):
         ctx.gemm3_grouped([t])
 
 
-# ---- split-precision form (NRG_GEMM_OZAKI): BF16 slices on the matrix cores, FP32 exact partial
+# ---- split-precision form (NRG_GEMM_OZAKI): INT8 slices on the matrix cores, exact INT32 partial
 # sums, FP64 accumulation. Bar: normwise ||C - C_exact||_F <= 1e-12 * sum_terms |f| ||A|| ||O|| ||B||
 # (Frobenius norms), the C4 criterion ||C - C_64||_F / (||U_I|| ||O|| ||U_J||) <= 1e-12.
 
