@@ -50,6 +50,21 @@ __device__ __forceinline__ void gemm_tile(const GemmDescT<T> &G, int tile, T *ld
   for (int i = 0; i < NF; i++)
 #pragma unroll
     for (int j = 0; j < NF; j++) acc[i][j] = (typename Acc<T>::type){0, 0, 0, 0};
+  // C's old values (beta != 0) are loaded before the K loop on the small tile, so their latency hides
+  // behind it (the big tile has no registers to spare and a K loop long enough to amortise them)
+  constexpr bool PRE = TM == 64;
+  const bool rmw = G.beta != 0.0;
+  typename Acc<T>::type cold[NF][NF];
+#pragma unroll
+  for (int i = 0; i < NF; i++)
+#pragma unroll
+    for (int j = 0; j < NF; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + i * 16 + (lane & 15);
+        const int n = n0 + wn + j * 16 + out_row<T>(lane, r);
+        cold[i][j][r] = (PRE && rmw && m < G.M && n < G.N) ? G.C[(size_t)n * G.ldc + m] : (T)0;
+      }
   auto fetch = [&](const T *A, const T *B, int lda, int ldb, int tA, int tB, int Kp, int kb, T sc, T (&ra)[PT],
                    T (&rb)[PT]) {
 #pragma unroll
@@ -175,9 +190,9 @@ __device__ __forceinline__ void gemm_tile(const GemmDescT<T> &G, int tile, T *ld
         const int m = m0 + wm + i * 16 + (lane & 15);
         const int n = n0 + wn + j * 16 + out_row<T>(lane, r);
         if (m < G.M && n < G.N) {
-          T *c = G.C + (size_t)n * G.ldc + m;
           const T v = (T)G.alpha * acc[i][j][r];
-          *c = G.beta == 0.0 ? v : v + (T)G.beta * *c;
+          T *c = G.C + (size_t)n * G.ldc + m;
+          *c = rmw ? v + (T)G.beta * (PRE ? cold[i][j][r] : *c) : v;
         }
       }
 }
