@@ -173,9 +173,10 @@ __global__ __launch_bounds__(256) void k_symm(SyevArgs<T> a, const int2 *items, 
 constexpr int KCH = 16;   // panel columns loaded per batch
 constexpr int NT_YR = 256;   // rows per workgroup of k_col_yred
 
-// RPT: rows per thread, 1 when every block of the launch has at most NT_PANEL rows [jl, m)
-template <typename T, int RPT>
-__global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int *blist, int jl) {
+// RPT: rows per thread, 1 when every block of the launch has at most NT rows [jl, m); NT threads
+// (fewer than NT_PANEL when the launch's blocks have few rows: fewer waves per barrier and reduction)
+template <typename T, int RPT, int NT = NT_PANEL>
+__global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist, int jl) {
   const int ent = blist[blockIdx.x], p = ent >> COL_PANEL_SHIFT;
   const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
   const int n = (int)bd.n;
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   const bool do_col = jl < w || final;           // column jl is updated here (and reduced if jl < w)
   const int jp = jl - 1;                         // reflector whose W column is finished here
   const int tid = threadIdx.x;
-  constexpr int NW = NT_PANEL / WAVE;
+  constexpr int NW = NT / WAVE;
   constexpr int KC = KCH;
 
   __shared__ T part[NW][64];
@@ -205,8 +206,8 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
 
   const T tv = jl > 0 ? tau[jp] : (T)0;
   const int Tt = (m - jl + 63) / 64;             // 64-row segments of the rows [jl, m)
-  T cf[RPT];   // column jl after its update, rows jl + tid + q NT_PANEL
-  if (m - jl > NT_PANEL) {
+  T cf[RPT];   // column jl after its update, rows jl + tid + q NT
+  if (m - jl > NT) {
     T v[RPT], y[RPT], aj[RPT];
     T *c = cf;
     // more rows than threads. The first column of a panel is only loaded. Otherwise k_col_yred has
@@ -216,18 +217,18 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     if (jl == 0) {
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
-        const int r = tid + q * NT_PANEL;
+        const int r = tid + q * NT;
         c[q] = r < m ? A[r] : (T)0;
       }
     } else {
     const T *cs = a.cs + bd.offV + i0;
     const T *dpp = a.dpp + bd.offV;
-    const int nch = (m - jl + NT_YR - 1) / NT_YR;   // at most NMAX / NT_YR <= NT_PANEL chunks
+    const int nch = (m - jl + NT_YR - 1) / NT_YR;   // at most NMAX / NT_YR <= NT chunks
     if (tid == 0) sh[1] = do_col ? A[(size_t)jp * n + jl] : (T)0;   // V(jl, jp)
-    const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(tid < nch ? dpp[tid] : (T)0, red);
+    const T a2 = (T)(-0.5) * tv * block_sum<NT>(tid < nch ? dpp[tid] : (T)0, red);
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-      const int r = jl + tid + q * NT_PANEL;
+      const int r = jl + tid + q * NT;
       v[q] = y[q] = c[q] = aj[q] = 0;
       if (r < m) {
         v[q] = vb[r];
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     __syncthreads();   // sh[1] is reused by the reflector below
     }
   } else {
-  // at most NT_PANEL rows [jl, m): one row per thread
+  // at most NT rows [jl, m): one row per thread
   constexpr int R1 = 1;
   T v[R1], y[R1], c[R1], aj[R1];
   // 1. loads that depend on nothing computed here
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   }
 #pragma unroll
   for (int q = 0; q < R1; q++) {
-    const int r = jl + tid + q * NT_PANEL;
+    const int r = jl + tid + q * NT;
     v[q] = y[q] = c[q] = aj[q] = 0;
     if (r < m) {
       if (jl > 0) {
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   T ap[R1][KP], wp[R1][KP];
 #pragma unroll
   for (int q = 0; q < R1; q++) {
-    const int r = jl + tid + q * NT_PANEL;
+    const int r = jl + tid + q * NT;
 #pragma unroll
     for (int k = 0; k < KP; k++) {
       const bool in = r < m && k < jp;
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
     T dp = 0;
 #pragma unroll
     for (int q = 0; q < R1; q++) {
-      const int r = jl + tid + q * NT_PANEL;
+      const int r = jl + tid + q * NT;
       if (r < m) {
         T ay = 0, ac = 0;
 #pragma unroll
@@ -338,10 +339,10 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
         c[q] -= ac;
       }
     }
-    const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
+    const T a2 = (T)(-0.5) * tv * block_sum<NT>(dp, red);
 #pragma unroll
     for (int q = 0; q < R1; q++) {
-      const int r = jl + tid + q * NT_PANEL;
+      const int r = jl + tid + q * NT;
       if (r < m) {
         y[q] += a2 * v[q];
         W[(size_t)jp * n + r] = y[q];
@@ -371,11 +372,11 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   T nrm = 0;
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
-    const int r = jl + tid + q * NT_PANEL;
+    const int r = jl + tid + q * NT;
     if (r >= jl + 2 && r < m) nrm += cf[q] * cf[q];
     if (r == jl + 1) sh[1] = cf[q];
   }
-  const T xn2 = block_sum<NT_PANEL>(nrm, red);
+  const T xn2 = block_sum<NT>(nrm, red);
   const T alpha = sh[1];
   T beta, tn, scal = 1;
   if (xn2 == (T)0) {
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_col_prep(SyevArgs<T> a, const int 
   T *colj = A + (size_t)jl * n;
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
-    const int r = jl + tid + q * NT_PANEL;
+    const int r = jl + tid + q * NT;
     if (r < m) {
       if (r == jl) {
         colj[r] = cf[q];
@@ -2728,7 +2729,13 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
       if (c.yred_cnt)
         NRG_TIMED(ctx, sg, "k_col_yred", hipLaunchKernelGGL(k_col_yred<T>, grid(c.yred_cnt), dim3(NT_YR), 0, sg, aT,
                                                             (const int2 *)(I + c.yred_off), c.jl));
-      if (c.rows <= NT_PANEL)
+      if (c.rows <= 256)
+        NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1, 256>), grid(c.prep_cnt), dim3(256), 0, sg, aT,
+                                                            I + c.prep_off, c.jl));
+      else if (c.rows <= 512)
+        NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1, 512>), grid(c.prep_cnt), dim3(512), 0, sg, aT,
+                                                            I + c.prep_off, c.jl));
+      else if (c.rows <= NT_PANEL)
         NRG_TIMED(ctx, sg, "k_col_prep", hipLaunchKernelGGL((k_col_prep<T, 1>), grid(c.prep_cnt), dim3(NT_PANEL), 0, sg, aT,
                                                             I + c.prep_off, c.jl));
       else if (c.rows <= 4 * NT_PANEL)
