@@ -240,7 +240,10 @@ def bench_eig(args):
                 "note": "algorithmic flops / call wall time / FP64 peak; the per-launch HBM model of the dominant "
                         "kernel is roofline_dominant_kernel"}
     roofdom = None
-    rb = reduction_bytes(sizes, groups=red_groups())
+    # launch counts follow the chain count the calls ran with: the autotuner's choice for this batch's
+    # signature, else NRG_RED_STREAMS or the default
+    chosen = next(iter(tuning.values()))["chains"] if tuning else -1
+    rb = reduction_bytes(sizes, groups=chosen if chosen > 0 else red_groups())
     if dom in rb:
         tot_bytes, nl = rb[dom]
         tms, cnt = kt[dom]

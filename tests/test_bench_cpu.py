@@ -34,9 +34,13 @@ def test_panel_constants_match_the_library():
 
 def test_launch_model_matches_the_trace():
     """Launches per call of the two column kernels on the 256-block workload, as the kernel trace in
-    profiles/syev_fp64_256blk_kernels.md counts them (14784 and 14336 launches over 8 calls with two
-    chains): the roofline's bytes per launch divide by these counts."""
+    profiles/syev_fp64_256blk_kernels.md counts them (22176 preparation launches, over three workgroup
+    sizes, and 21504 product launches over 8 calls with three chains; an earlier two-chain trace had
+    14784 and 14336): the roofline's bytes per launch divide by these counts."""
     sizes = workloads.eig_sizes()
+    three = bench.reduction_bytes(sizes, groups=3)
+    assert three["k_col_prep"][1] == 22176 // 8
+    assert three["k_col_symv"][1] == 21504 // 8
     two = bench.reduction_bytes(sizes, groups=2)
     assert two["k_col_prep"][1] == 14784 // 8
     assert two["k_col_symv"][1] == 14336 // 8
