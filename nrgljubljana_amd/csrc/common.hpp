@@ -25,20 +25,86 @@ void set_error(const std::string &msg);
   } while (0)
 
 // ---------------------------------------------------------------- device reductions
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+// Cross-lane moves that do not go through the LDS: DPP permutations within rows of 16 lanes and the
+// permlane swaps between rows and between wave halves (gfx950). Lane semantics, as checked on the GPU:
+//   DPP_MIRROR: lane i of a row reads lane 15 - i; DPP_HALF_MIRROR: lane i of 8 reads lane 7 - i;
+//   DPP_QXOR2 / DPP_QXOR1: lane i of a quad reads lane i ^ 2 / i ^ 1;
+//   swap16(a, b): a <- rows (a0, b0, a2, b2), b <- rows (a1, b1, a3, b3);
+//   swap32(a, b): a <- halves (a_lo, b_lo), b <- halves (a_hi, b_hi).
+constexpr int DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141, DPP_QXOR2 = 0x4E, DPP_QXOR1 = 0xB1;
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp32(unsigned v) { return __builtin_amdgcn_update_dpp(0u, v, CTRL, 0xF, 0xF, true); }
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  u32x2_t u = __builtin_bit_cast(u32x2_t, v);
+  u[0] = dpp32<CTRL>(u[0]);
+  u[1] = dpp32<CTRL>(u[1]);
+  return __builtin_bit_cast(double, u);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) { return __builtin_bit_cast(float, dpp32<CTRL>(__builtin_bit_cast(unsigned, v))); }
+__device__ __forceinline__ void swap16(unsigned &a, unsigned &b) {
+  const u32x2_t r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+  a = r[0];
+  b = r[1];
+}
+__device__ __forceinline__ void swap32(unsigned &a, unsigned &b) {
+  const u32x2_t r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+  a = r[0];
+  b = r[1];
+}
+template <bool HALF32>
+__device__ __forceinline__ void swap_rows(double &a, double &b) {
+  u32x2_t ua = __builtin_bit_cast(u32x2_t, a), ub = __builtin_bit_cast(u32x2_t, b);
+  unsigned a0 = ua[0], a1 = ua[1], b0 = ub[0], b1 = ub[1];
+  if (HALF32) { swap32(a0, b0); swap32(a1, b1); } else { swap16(a0, b0); swap16(a1, b1); }
+  ua[0] = a0; ua[1] = a1; ub[0] = b0; ub[1] = b1;
+  a = __builtin_bit_cast(double, ua);
+  b = __builtin_bit_cast(double, ub);
+}
+template <bool HALF32>
+__device__ __forceinline__ void swap_rows(float &a, float &b) {
+  unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+  if (HALF32) swap32(ua, ub); else swap16(ua, ub);
+  a = __builtin_bit_cast(float, ua);
+  b = __builtin_bit_cast(float, ub);
+}
+// Sum over each row of 16 lanes, in every lane of the row. Every step adds a lane and its partner in
+// both, so all lanes of a row get the same bits.
+template <typename T>
+__device__ __forceinline__ T row16_sum(T v) {
+  v += dpp<DPP_QXOR1>(v);
+  v += dpp<DPP_QXOR2>(v);
+  v += dpp<DPP_HALF_MIRROR>(v);
+  v += dpp<DPP_MIRROR>(v);
   return v;
 }
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
+template <typename T>
+__device__ __forceinline__ T wave_sum_t(T v) {
+  v = row16_sum(v);
+  T a = v, b = v;
+  swap_rows<false>(a, b);   // a + b: rows 0 + 1 in rows 0, 1; rows 2 + 3 in rows 2, 3
+  v = a + b;
+  a = v;
+  b = v;
+  swap_rows<true>(a, b);    // the two halves
+  return a + b;
 }
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_t(v); }
+__device__ __forceinline__ float wave_sum(float v) { return wave_sum_t(v); }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
-  return v;
+  v = fmax(v, dpp<DPP_QXOR1>(v));
+  v = fmax(v, dpp<DPP_QXOR2>(v));
+  v = fmax(v, dpp<DPP_HALF_MIRROR>(v));
+  v = fmax(v, dpp<DPP_MIRROR>(v));
+  double a = v, b = v;
+  swap_rows<false>(a, b);
+  v = fmax(a, b);
+  a = v;
+  b = v;
+  swap_rows<true>(a, b);
+  return fmax(a, b);
 }
 
 // Sum over a workgroup of NT threads. `red` holds NT/64 entries of LDS. All threads get the sum.

@@ -412,46 +412,43 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
 }
 
 // Sum over the 64 lanes of x[c(lane)], c(lane) = 8 b5 + 4 b4 + 2 b3 + b2 (bits of the lane id):
-// a butterfly that halves the number of live values per lane at every exchange (17 shuffles).
+// a butterfly that halves the number of live values per lane at every exchange, without the LDS
+// (permlane swaps across the wave halves and the rows, DPP mirrors within rows and quads).
 template <typename T>
 __device__ __forceinline__ T butterfly16(T (&x)[16]) {
   const int l = threadIdx.x & (WAVE - 1);
-  {
-    const bool up = l & 32;
+  // bit 5: lanes 0-31 keep x[j] + x[j] of lane + 32, lanes 32-63 keep x[j + 8] + x[j + 8] of lane - 32
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const T send = up ? x[j] : x[j + 8];
-      const T keep = up ? x[j + 8] : x[j];
-      x[j] = keep + __shfl_xor(send, 32, WAVE);
-    }
+  for (int j = 0; j < 8; j++) {
+    T a = x[j], b = x[j + 8];
+    swap_rows<true>(a, b);
+    x[j] = a + b;
   }
-  {
-    const bool up = l & 16;
+  // bit 4, rows 0/2 against rows 1/3
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const T send = up ? x[j] : x[j + 4];
-      const T keep = up ? x[j + 4] : x[j];
-      x[j] = keep + __shfl_xor(send, 16, WAVE);
-    }
+  for (int j = 0; j < 4; j++) {
+    T a = x[j], b = x[j + 4];
+    swap_rows<false>(a, b);
+    x[j] = a + b;
   }
-  {
+  {   // bit 3: the partner lane 15 - i of the row has the other bit 3
     const bool up = l & 8;
 #pragma unroll
     for (int j = 0; j < 2; j++) {
       const T send = up ? x[j] : x[j + 2];
       const T keep = up ? x[j + 2] : x[j];
-      x[j] = keep + __shfl_xor(send, 8, WAVE);
+      x[j] = keep + dpp<DPP_MIRROR>(send);
     }
   }
-  {
+  {   // bit 2: the partner lane 7 - i of the eight has the other bit 2
     const bool up = l & 4;
     const T send = up ? x[0] : x[1];
     const T keep = up ? x[1] : x[0];
-    x[0] = keep + __shfl_xor(send, 4, WAVE);
+    x[0] = keep + dpp<DPP_HALF_MIRROR>(send);
   }
   T s = x[0];
-  s += __shfl_xor(s, 2, WAVE);
-  s += __shfl_xor(s, 1, WAVE);
+  s += dpp<DPP_QXOR2>(s);
+  s += dpp<DPP_QXOR1>(s);
   return s;
 }
 
@@ -927,8 +924,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *bli
         T y = 0;
         if (rr < mt)
           for (int c = sub; c < mt; c += 16) y += S[(r1 + c) * LD + r1 + rr] * vv[r1 + c];
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) y += __shfl_xor(y, o, WAVE);
+        y = row16_sum(y);   // the 16 threads of a row are a row of 16 lanes
         if (rr < mt && sub == 0) {
           y *= tv;
           yy[r1 + rr] = y;
@@ -1062,15 +1058,11 @@ __device__ __forceinline__ int *isa(const SyevArgs<double> &a, const BlkDesc &bd
 // the two adjacent poles exact and the rest frozen; then Newton on the function with both
 // adjacent poles multiplied out, safeguarded by bisection.
 // A group of SL consecutive lanes solves one root: every sum over the poles is split across the
-// group (lane `sub` takes i = sub mod SL) and combined with an xor butterfly, which gives every
-// lane of the group the same bits (the pairwise additions commute), so the group's lanes take the
-// same iterations and branches.
-constexpr int SL = 16;   // lanes per secular root
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int o = SL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
-}
+// group (lane `sub` takes i = sub mod SL) and combined pairwise within the row (row16_sum), which
+// gives every lane of the group the same bits (the pairwise additions commute), so the group's lanes
+// take the same iterations and branches.
+constexpr int SL = 16;   // lanes per secular root: a row of 16 lanes
+__device__ __forceinline__ double group_sum(double v) { return row16_sum(v); }
 
 __device__ void secular_root(int j, int k, const double *d, const double *z, double rho, double zz, int sub, int &org,
                              double &tau_out) {
