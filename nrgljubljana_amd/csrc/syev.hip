@@ -225,7 +225,8 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
     const T *dpp = a.dpp + bd.offV;
     const int nch = (m - jl + NT_YR - 1) / NT_YR;   // at most NMAX / NT_YR <= NT chunks
     if (tid == 0) sh[1] = do_col ? A[(size_t)jp * n + jl] : (T)0;   // V(jl, jp)
-    const T a2 = (T)(-0.5) * tv * block_sum<NT>(tid < nch ? dpp[tid] : (T)0, red);
+    // no closing barrier: the one below, before the reflector, comes first
+    const T a2 = (T)(-0.5) * tv * block_sum<NT, false>(tid < nch ? dpp[tid] : (T)0, red);
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int r = jl + tid + q * NT;
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
         c[q] -= ac;
       }
     }
-    const T a2 = (T)(-0.5) * tv * block_sum<NT>(dp, red);
+    const T a2 = (T)(-0.5) * tv * block_sum<NT, false>(dp, red);   // the barrier after W(:,jp) follows
 #pragma unroll
     for (int q = 0; q < R1; q++) {
       const int r = jl + tid + q * NT;
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
     if (r >= jl + 2 && r < m) nrm += cf[q] * cf[q];
     if (r == jl + 1) sh[1] = cf[q];
   }
-  const T xn2 = block_sum<NT>(nrm, red);
+  const T xn2 = block_sum<NT, false>(nrm, red);   // red is not written again
   const T alpha = sh[1];
   T beta, tn, scal = 1;
   if (xn2 == (T)0) {
@@ -862,8 +863,9 @@ __global__ __launch_bounds__(WGR_MAX) void k_wg_red(SyevArgs<T> a, const int *bl
 //     (the whole block when n <= TAIL) is reduced column by column (dsytd2, lower) in LDS by one
 //     workgroup, with the reflectors, tau, d and e stored as the column kernels store them.
 // ------------------------------------------------------------------------------------------
+constexpr int NT_TAIL = NT_PANEL;   // threads of k_tail (256 took 0.68 ms per call against 0.49)
 template <typename T>
-__global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *blist) {
+__global__ __launch_bounds__(NT_TAIL) void k_tail(SyevArgs<T> a, const int *blist) {
   const int ent = blist[blockIdx.x], p = ent >> COL_PANEL_SHIFT;
   const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
   const int n = (int)bd.n;
@@ -874,12 +876,12 @@ __global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *bli
   double *e = a.e + bd.offV + i0;
   T *tau = a.tau + bd.offV + i0;
   constexpr int LD = TAIL + 1;
-  constexpr int NW = NT_PANEL / WAVE;
+  constexpr int NW = NT_TAIL / WAVE;
   __shared__ T S[TAIL * LD];        // S(r,c) at S[c*LD + r], full symmetric
   __shared__ T vv[TAIL], yy[TAIL];
   __shared__ T red[NW];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < m * m; idx += NT_PANEL) {
+  for (int idx = tid; idx < m * m; idx += NT_TAIL) {
     const int r = idx % m, c = idx / m;
     if (r >= c) {
       const T x = A[(size_t)c * n + r];
@@ -892,8 +894,8 @@ __global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *bli
     // reflector annihilating S(j+2:m, j) (dlarfg)
     const int r1 = j + 1;
     T part = 0;
-    for (int r = r1 + 1 + tid; r < m; r += NT_PANEL) part += S[j * LD + r] * S[j * LD + r];
-    const T xn2 = block_sum<NT_PANEL>(part, red);
+    for (int r = r1 + 1 + tid; r < m; r += NT_TAIL) part += S[j * LD + r] * S[j * LD + r];
+    const T xn2 = block_sum<NT_TAIL, false>(part, red);   // the barrier after the reflector follows
     const T alpha = S[j * LD + r1];
     T beta, tv, scal = 1;
     if (xn2 == (T)0) {
@@ -904,7 +906,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *bli
       tv = (beta - alpha) / beta;
       scal = (T)1 / (alpha - beta);
     }
-    for (int r = r1 + tid; r < m; r += NT_PANEL) {
+    for (int r = r1 + tid; r < m; r += NT_TAIL) {
       const T x = r == r1 ? (T)1 : S[j * LD + r] * scal;
       vv[r] = x;
       A[(size_t)j * n + r] = x;
@@ -919,7 +921,7 @@ __global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *bli
       // y = tau S22 v over the rows [r1, m): 16 threads per row
       const int mt = m - r1;
       T dp = 0;
-      for (int base = 0; base < mt; base += NT_PANEL / 16) {
+      for (int base = 0; base < mt; base += NT_TAIL / 16) {
         const int rr = base + tid / 16, sub = tid % 16;
         T y = 0;
         if (rr < mt)
@@ -931,9 +933,9 @@ __global__ __launch_bounds__(NT_PANEL) void k_tail(SyevArgs<T> a, const int *bli
           dp += y * vv[r1 + rr];
         }
       }
-      const T a2 = (T)(-0.5) * tv * block_sum<NT_PANEL>(dp, red);
+      const T a2 = (T)(-0.5) * tv * block_sum<NT_TAIL, false>(dp, red);   // the barrier after the update follows
       // w = y + a2 v; S22 -= v w^T + w v^T
-      for (int idx = tid; idx < mt * mt; idx += NT_PANEL) {
+      for (int idx = tid; idx < mt * mt; idx += NT_TAIL) {
         const int rr = r1 + idx % mt, cc = r1 + idx / mt;
         const T wr = yy[rr] + a2 * vv[rr], wc = yy[cc] + a2 * vv[cc];
         S[cc * LD + rr] -= vv[rr] * wc + wr * vv[cc];
@@ -2080,11 +2082,13 @@ struct SyevGraph {
 // measured (it pays for first-use costs); the next TUNE_ROUNDS x NTUNE calls run the candidates
 // (TUNE_CAND chains) in turn, each measured by the call's own event time per unit of work (Σ n³, so
 // that the different steps of an NRG run with one signature compare); then the candidate with the
-// smallest time per unit is used for that signature. NRG_RED_STREAMS fixes the count, NRG_AUTOTUNE=0
-// keeps RED_GROUPS.
+// smallest time per unit is used for that signature if it beats the default by TUNE_MARGIN.
+// NRG_RED_STREAMS fixes the count, NRG_AUTOTUNE=0 keeps RED_GROUPS.
 constexpr int TUNE_CAND[] = {2, 3, 4};
 constexpr int NTUNE = sizeof(TUNE_CAND) / sizeof(int);
 constexpr int TUNE_ROUNDS = 2;
+constexpr double TUNE_MARGIN = 0.02;
+static_assert(TUNE_CAND[0] == RED_GROUPS, "the first candidate is the default");
 struct TuneEntry {
   double unit[NTUNE] = {};   // least ms per 1e9 n^3 measured for each candidate
   float ms[NTUNE] = {};      // call time of that measurement
@@ -2170,9 +2174,11 @@ static void tune_record(SyevCache &c, const TuneSig &sig, int cand, float ms, in
     }
   }
   if (++t.step == TUNE_ROUNDS * NTUNE) {
+    // the default (TUNE_CAND[0] = RED_GROUPS) unless another count is at least TUNE_MARGIN faster:
+    // smaller differences are within the call-to-call spread, and more chains spread more
     t.best = 0;
     for (int k = 1; k < NTUNE; k++)
-      if (t.unit[k] < t.unit[t.best]) t.best = k;
+      if (t.unit[k] < t.unit[t.best] && t.unit[k] < (1.0 - TUNE_MARGIN) * t.unit[0]) t.best = k;
   }
 }
 
@@ -2759,7 +2765,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     NRG_HIP(hipStreamWaitEvent(st, ctx->join_ev[nrg_ctx::NAUX - 1], 0));
   }
   if (pl.tail_cnt)
-    NRG_TIMED(ctx, st, "k_tail", hipLaunchKernelGGL(k_tail<T>, grid(pl.tail_cnt), dim3(NT_PANEL), 0, st, aT, I + pl.tail_off));
+    NRG_TIMED(ctx, st, "k_tail", hipLaunchKernelGGL(k_tail<T>, grid(pl.tail_cnt), dim3(NT_TAIL), 0, st, aT, I + pl.tail_off));
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipEventRecord(ctx->ev[phase0 + 1], st));
   // 3. divide and conquer

@@ -378,6 +378,8 @@ def test_autotuner_measures_then_chooses(ctx, monkeypatch):
     (entry,) = table.values()
     assert entry["calls"] == 9 and entry["chains"] in (2, 3, 4) and len(entry["ms_per_1e9_n3"]) == 3
     unit = entry["ms_per_1e9_n3"]
-    assert unit[str(entry["chains"])] == min(unit.values())
+    # the fastest count, unless it is within 2 % of the default two chains
+    best = min(unit, key=unit.get)
+    assert str(entry["chains"]) == (best if unit[best] < 0.98 * unit["2"] else "2")
     for wk, Vk in results[1:]:
         assert np.array_equal(wk, results[0][0]) and np.array_equal(Vk, results[0][1])
