@@ -108,7 +108,21 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Sum over a workgroup of NT threads. `red` holds NT/64 entries of LDS. All threads get the sum.
-template <int NT, typename T>
+// 1/x to about an ulp: the hardware reciprocal estimate (v_rcp_f64) refined by two Newton-Raphson
+// steps (e = 1 - x r, r += r e), about half the instructions of an IEEE division. For the O(k^2)
+// terms of the divide-and-conquer kernels, whose sums and products tolerate an ulp per term; x must
+// be a finite normal number (their denominators are separations of distinct poles and roots).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// TRAIL = false leaves out the closing barrier, for callers that pass another barrier before `red`
+// is written again.
+template <int NT, bool TRAIL = true, typename T>
 __device__ __forceinline__ T block_sum(T v, T *red) {
   v = wave_sum(v);
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
@@ -117,7 +131,7 @@ __device__ __forceinline__ T block_sum(T v, T *red) {
   T s = 0;
 #pragma unroll
   for (int i = 0; i < NT / WAVE; i++) s += red[i];
-  __syncthreads();
+  if (TRAIL) __syncthreads();
   return s;
 }
 template <int NT>

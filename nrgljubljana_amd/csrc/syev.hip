@@ -1076,7 +1076,7 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
     double R0 = 0.0;
     for (int i = sub; i < k; i += SL) {
       if (i == j || i == j + 1) continue;
-      R0 += rho * z[i] * z[i] / ((d[i] - d[j]) - mid);
+      R0 += rho * z[i] * z[i] * rcp_nr((d[i] - d[j]) - mid);
     }
     R0 = group_sum(R0);
     const double f = 1.0 + R0 - rho * z[j] * z[j] / mid + rho * z[j + 1] * z[j + 1] / (g - mid);
@@ -1098,7 +1098,7 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
     za = rho * z[o] * z[o];
     const double t0 = 0.5 * hi;
     double R0 = 0.0;
-    for (int i = sub; i < k - 1; i += SL) R0 += rho * z[i] * z[i] / ((d[i] - d[o]) - t0);
+    for (int i = sub; i < k - 1; i += SL) R0 += rho * z[i] * z[i] * rcp_nr((d[i] - d[o]) - t0);
     R0 = group_sum(R0);
     t = t0;
     if (1.0 + R0 > 0) {
@@ -1113,14 +1113,14 @@ __device__ void secular_root(int j, int k, const double *d, const double *z, dou
     double R = 0, Rp = 0, Ra = 0, R2 = 0, Rp2 = 0, Ra2 = 0;
     int i = sub;
     for (; i + SL < k; i += 2 * SL) {
-      const double q0 = 1.0 / ((d[i] - dorg) - t), q1 = 1.0 / ((d[i + SL] - dorg) - t);
+      const double q0 = rcp_nr((d[i] - dorg) - t), q1 = rcp_nr((d[i + SL] - dorg) - t);
       const double t0 = (i == o || i == o2) ? 0.0 : rho * z[i] * z[i] * q0;
       const double t1 = (i + SL == o || i + SL == o2) ? 0.0 : rho * z[i + SL] * z[i + SL] * q1;
       R += t0; Rp += t0 * q0; Ra += fabs(t0);
       R2 += t1; Rp2 += t1 * q1; Ra2 += fabs(t1);
     }
     if (i < k) {
-      const double q0 = 1.0 / ((d[i] - dorg) - t);
+      const double q0 = rcp_nr((d[i] - dorg) - t);
       const double t0 = (i == o || i == o2) ? 0.0 : rho * z[i] * z[i] * q0;
       R += t0; Rp += t0 * q0; Ra += fabs(t0);
     }
@@ -1383,7 +1383,7 @@ __global__ __launch_bounds__(256) void k_dc_zhat(SyevArgs<double> a, const int2 
     double W = (di - ol[i]) - tl[i];
     for (int j = 0; j < k; j++) {
       if (j == i) continue;
-      W *= ((di - ol[j]) - tl[j]) / (di - dl[j]);
+      W *= ((di - ol[j]) - tl[j]) * rcp_nr(di - dl[j]);
     }
     dsa(a, bd, DS_ZHAT, g.a0)[i] = copysign(sqrt(fabs(W)), dsa(a, bd, DS_ZND, g.a0)[i]);
   }
@@ -1436,7 +1436,7 @@ __global__ __launch_bounds__(256) void k_dc_normrank(SyevArgs<double> a, const i
       const double dj = dorg[u], tj = rt[u];
       double s = 0;
       for (int i = 0; i < k; i++) {
-        const double v = zl[i] / ((dl[i] - dj) - tj);
+        const double v = zl[i] * rcp_nr((dl[i] - dj) - tj);
         s += v * v;
       }
       dsa(a, bd, DS_INVN, g.a0)[u] = 1.0 / sqrt(s);
@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(256) void k_dc_fill(SyevArgs<double> a, const int2 
     double val;
     if (u < k) {
       const int cix = cmp[p];
-      val = cix < 0 ? 0.0 : zl[clampi(cix, k)] / ((dl[clampi(cix, k)] - dorg[u]) - rt[u]) * invn[u];
+      val = cix < 0 ? 0.0 : zl[clampi(cix, k)] * rcp_nr((dl[clampi(cix, k)] - dorg[u]) - rt[u]) * invn[u];
     } else {
       val = (p == clampi(cpos[u], m)) ? 1.0 : 0.0;
     }
