@@ -1177,12 +1177,17 @@ inline int dc_deflate_lds(int mc) { return (int)(2 * sizeof(double) * (size_t)mc
 // dynamic LDS a merge kernel may ask for: the 160 KiB of a workgroup less room for static arrays
 constexpr int DYN_LDS_MAX = 160 * 1024 - 8 * 1024;
 
-// gscr != nullptr: merges too large for LDS keep d, z, predecessors and flags at gscr + merge * gstride
-// (global memory, one region per workgroup) instead of in dynamic LDS
+// G: merges too large for LDS keep d, z, predecessors and flags at gscr + merge * gstride (global
+// memory, one region per workgroup) instead of in dynamic LDS, through generic pointers. With the
+// scratch in LDS (G = false) the compiler sees LDS loads and global stores (dcg pointers), so the
+// serial scan's LDS loads do not wait for its stores of the merge's lists.
+template <bool G, typename T>
+using dcg = std::conditional_t<G, T *, __attribute__((address_space(1))) T *>;
+template <bool G>
 __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const int2 *items, int depth,
                                                        MergeState *mst, double *gscr, int64_t gstride) {
   extern __shared__ double dyn_lds[];
-  double *dyn = gscr ? gscr + (size_t)blockIdx.x * gstride : dyn_lds;
+  double *dyn = G ? (gscr ? gscr + (size_t)blockIdx.x * gstride : dyn_lds) : dyn_lds;
   __shared__ double red[NT_DEF / WAVE];
   const int2 it = items[blockIdx.x];
   const BlkDesc bd = a.bd[it.x];
@@ -1197,13 +1202,15 @@ __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const
   const double sgn = beta >= 0 ? 1.0 : -1.0;
   const double rho = 2.0 * fabs(beta);
   const double r2 = 0.70710678118654752440;
-  double *dcat = dsa(a, bd, DS_DCAT, a0), *zcat = dsa(a, bd, DS_ZCAT, a0);
-  double *ds = dsa(a, bd, DS_DS, a0);
-  double *dnd = dsa(a, bd, DS_DND, a0), *znd = dsa(a, bd, DS_ZND, a0);
-  double *rc = dsa(a, bd, DS_RC, a0), *rs = dsa(a, bd, DS_RS, a0);
-  int *perm = isa(a, bd, IS_PERM, a0), *pinv = isa(a, bd, IS_PINV, a0);
-  int *cmp = isa(a, bd, IS_CMP, a0), *cpos = isa(a, bd, IS_CPOS, a0);
-  int *rpj = isa(a, bd, IS_RPJ, a0), *rnj = isa(a, bd, IS_RNJ, a0);
+  using GD = dcg<G, double>;
+  using GI = dcg<G, int>;
+  GD dcat = (GD)dsa(a, bd, DS_DCAT, a0), zcat = (GD)dsa(a, bd, DS_ZCAT, a0);
+  GD ds = (GD)dsa(a, bd, DS_DS, a0);
+  GD dnd = (GD)dsa(a, bd, DS_DND, a0), znd = (GD)dsa(a, bd, DS_ZND, a0);
+  GD rc = (GD)dsa(a, bd, DS_RC, a0), rs = (GD)dsa(a, bd, DS_RS, a0);
+  GI perm = (GI)isa(a, bd, IS_PERM, a0), pinv = (GI)isa(a, bd, IS_PINV, a0);
+  GI cmp = (GI)isa(a, bd, IS_CMP, a0), cpos = (GI)isa(a, bd, IS_CPOS, a0);
+  GI rpj = (GI)isa(a, bd, IS_RPJ, a0), rnj = (GI)isa(a, bd, IS_RNJ, a0);
   // the two sorted spectra and z = (last row of Q1, +-first row of Q2) / sqrt(2)
   for (int i = tid; i < m; i += NT_DEF) {
     dcat[i] = lam[a0 + i];
@@ -1213,8 +1220,8 @@ __global__ __launch_bounds__(NT_DEF) void k_dc_deflate(SyevArgs<double> a, const
   double mx = 0;
   for (int i = tid; i < m; i += NT_DEF) {
     int pos;
-    if (i < m1) pos = i + lower_bound_d(dcat + m1, m2, dcat[i]);
-    else pos = (i - m1) + upper_bound_d(dcat, m1, dcat[i]);
+    if (i < m1) pos = i + lower_bound_d((const double *)(dcat + m1), m2, dcat[i]);
+    else pos = (i - m1) + upper_bound_d((const double *)dcat, m1, dcat[i]);
     pos = clampi(pos, m);
     perm[pos] = i;
     pinv[i] = pos;
@@ -2773,7 +2780,7 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     int mc = 0;
     for (int depth = 0; depth < pl.Lmax; depth++) mc = std::max(mc, pl.mcap[depth]);
     const int lds3 = (int)(3 * sizeof(double) * (size_t)mc + 16), lds2 = (int)(2 * sizeof(double) * (size_t)mc + 16);
-    (void)hipFuncSetAttribute((const void *)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void *)k_dc_deflate<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               std::min(dc_deflate_lds(mc), DYN_LDS_MAX));
     (void)hipFuncSetAttribute((const void *)k_dc_roots, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
     (void)hipFuncSetAttribute((const void *)k_dc_zhat<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2806,9 +2813,13 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
     const int2 *ch = (const int2 *)(I + pl.ch_off[depth]);
     const int2 *fc = (const int2 *)(I + pl.fc_off[depth]);
     const bool dg = dc_deflate_lds(mc) > DYN_LDS_MAX, lg = (int)lds3 > DYN_LDS_MAX;
-    NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate, grid(pl.mrg_cnt[depth]), dim3(NT_DEF),
-                                                          dg ? 0 : dc_deflate_lds(mc), st, aD, mi, depth, L.mst,
-                                                          dg ? dc_gscr : nullptr, dc_gstride));
+    if (dg)
+      NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate<true>, grid(pl.mrg_cnt[depth]), dim3(NT_DEF), 0,
+                                                            st, aD, mi, depth, L.mst, dc_gscr, dc_gstride));
+    else
+      NRG_TIMED(ctx, st, "k_dc_deflate", hipLaunchKernelGGL(k_dc_deflate<false>, grid(pl.mrg_cnt[depth]), dim3(NT_DEF),
+                                                            dc_deflate_lds(mc), st, aD, mi, depth, L.mst, nullptr,
+                                                            dc_gstride));
     NRG_TIMED(ctx, st, "k_dc_roots", hipLaunchKernelGGL(k_dc_roots, grid(pl.rc_cnt[depth]), dim3(256), lds2, st, aD, mi,
                                                         (const int2 *)(I + pl.rc_off[depth]), depth, (const MergeState *)L.mst));
     if (lg) {
