@@ -605,14 +605,8 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
   T *yp = a.Yp + bd.offYp + i0 + r0;             // yp[slot*n + r]
   __shared__ T vI[64], vJ[64];
   __shared__ T red[4][64];
-  if (tid < 64) {
-    const int r = I * 64 + tid;
-    vI[tid] = r < t ? v[r] : (T)0;
-  } else if (tid < 128) {
-    const int c = J * 64 + tid - 64;
-    vJ[tid - 64] = c < t ? v[c] : (T)0;
-  }
-  __syncthreads();
+  // the tile's loads first: they do not depend on the reflector, so their round trip overlaps the
+  // staging of v below
   const int r = I * 64 + lane;
   const bool rin = r < t;
   T x[16];
@@ -621,6 +615,14 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
     const int c = J * 64 + wv * 16 + q;
     x[q] = (rin && c < t) ? S[(size_t)c * n + r] : (T)0;
   }
+  if (tid < 64) {
+    const int rr = I * 64 + tid;
+    vI[tid] = rr < t ? v[rr] : (T)0;
+  } else if (tid < 128) {
+    const int c = J * 64 + tid - 64;
+    vJ[tid - 64] = c < t ? v[c] : (T)0;
+  }
+  __syncthreads();
   T yi = 0;
 #pragma unroll
   for (int q = 0; q < 16; q++) yi += x[q] * vJ[wv * 16 + q];
