@@ -1455,6 +1455,10 @@ __global__ __launch_bounds__(256) void k_dc_normrank(SyevArgs<double> a, const i
 
 constexpr int FILLC = 32;  // columns of S per fill workgroup
 
+// STAGE: the rows' positions pinv[r] and cmp[pinv[r]] and the chunk's column data go to LDS first, so
+// the element loop reads no global memory (merges whose arrays fit: fill_lds)
+inline int fill_lds(int mc) { return (int)(2 * sizeof(double) * (size_t)mc + 2 * sizeof(int) * (size_t)mc + 16); }
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_dc_fill(SyevArgs<double> a, const int2 *mitems, const int2 *citems, int depth,
                                                  const MergeState *mst) {
   extern __shared__ double dyn[];
@@ -1482,20 +1486,55 @@ __global__ __launch_bounds__(256) void k_dc_fill(SyevArgs<double> a, const int2 
     dl[i] = dnd[i];
     zl[i] = zhat[i];
   }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < nc * m; idx += 256) {
-    const int cc = idx / m, r = idx - cc * m;
-    const int c = c0 + cc;
-    const int u = clampi(colsrc[c], m);
-    const int p = clampi(pinv[r], m);
-    double val;
-    if (u < k) {
-      const int cix = cmp[p];
-      val = cix < 0 ? 0.0 : zl[clampi(cix, k)] * rcp_nr((dl[clampi(cix, k)] - dorg[u]) - rt[u]) * invn[u];
-    } else {
-      val = (p == clampi(cpos[u], m)) ? 1.0 : 0.0;
+  if (STAGE) {
+    int *pr = (int *)(zl + m), *cixr = pr + m;   // pinv[r] and cmp[pinv[r]] of every row
+    __shared__ int ucol[FILLC], ccpos[FILLC];
+    __shared__ double cdorg[FILLC], crt[FILLC], cinvn[FILLC];
+    for (int r = threadIdx.x; r < m; r += 256) {
+      const int p = clampi(pinv[r], m);
+      pr[r] = p;
+      cixr[r] = cmp[p];
     }
-    S[(size_t)(a0 + c) * ld + (a0 + r)] = val;
+    if (threadIdx.x < nc) {
+      const int cc = threadIdx.x, u = clampi(colsrc[c0 + cc], m);
+      ucol[cc] = u;
+      if (u < k) {
+        cdorg[cc] = dorg[u];
+        crt[cc] = rt[u];
+        cinvn[cc] = invn[u];
+      } else {
+        ccpos[cc] = clampi(cpos[u], m);
+      }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nc * m; idx += 256) {
+      const int cc = idx / m, r = idx - cc * m;
+      const int u = ucol[cc];
+      double val;
+      if (u < k) {
+        const int cix = cixr[r];
+        val = cix < 0 ? 0.0 : zl[clampi(cix, k)] * rcp_nr((dl[clampi(cix, k)] - cdorg[cc]) - crt[cc]) * cinvn[cc];
+      } else {
+        val = (pr[r] == ccpos[cc]) ? 1.0 : 0.0;
+      }
+      S[(size_t)(a0 + c0 + cc) * ld + (a0 + r)] = val;
+    }
+  } else {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nc * m; idx += 256) {
+      const int cc = idx / m, r = idx - cc * m;
+      const int c = c0 + cc;
+      const int u = clampi(colsrc[c], m);
+      const int p = clampi(pinv[r], m);
+      double val;
+      if (u < k) {
+        const int cix = cmp[p];
+        val = cix < 0 ? 0.0 : zl[clampi(cix, k)] * rcp_nr((dl[clampi(cix, k)] - dorg[u]) - rt[u]) * invn[u];
+      } else {
+        val = (p == clampi(cpos[u], m)) ? 1.0 : 0.0;
+      }
+      S[(size_t)(a0 + c) * ld + (a0 + r)] = val;
+    }
   }
   __syncthreads();
   // undo the deflation rotations (last generated first) on the rows of this chunk's columns
@@ -2789,7 +2828,9 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
                               std::min(lds3, DYN_LDS_MAX));
     (void)hipFuncSetAttribute((const void *)k_dc_normrank<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               std::min(lds3, DYN_LDS_MAX));
-    (void)hipFuncSetAttribute((const void *)k_dc_fill, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
+    (void)hipFuncSetAttribute((const void *)k_dc_fill<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds2);
+    (void)hipFuncSetAttribute((const void *)k_dc_fill<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              std::min(fill_lds(mc), DYN_LDS_MAX));
   }
   // merges whose deflation does not fit the LDS keep its arrays in global memory, one region each
   double *dc_gscr = nullptr;
@@ -2835,8 +2876,12 @@ static int launch_core(nrg_ctx *ctx, SyevPlan &pl, SyevArgs<T> &aT, SyevArgs<dou
       NRG_TIMED(ctx, st, "k_dc_normrank", hipLaunchKernelGGL(k_dc_normrank<true>, grid(pl.ch_cnt[depth]), dim3(256), lds3,
                                                              st, aD, mi, ch, depth, (const MergeState *)L.mst));
     }
-    NRG_TIMED(ctx, st, "k_dc_fill", hipLaunchKernelGGL(k_dc_fill, grid(pl.fc_cnt[depth]), dim3(256), lds2, st, aD, mi, fc,
-                                                       depth, (const MergeState *)L.mst));
+    if (fill_lds(mc) <= DYN_LDS_MAX)
+      NRG_TIMED(ctx, st, "k_dc_fill", hipLaunchKernelGGL(k_dc_fill<true>, grid(pl.fc_cnt[depth]), dim3(256), fill_lds(mc), st,
+                                                         aD, mi, fc, depth, (const MergeState *)L.mst));
+    else
+      NRG_TIMED(ctx, st, "k_dc_fill", hipLaunchKernelGGL(k_dc_fill<false>, grid(pl.fc_cnt[depth]), dim3(256), lds2, st, aD,
+                                                         mi, fc, depth, (const MergeState *)L.mst));
     if (pl.gemm_cnt[depth]) gemm_launch(ctx, st, L.dG, I + pl.gemm_off[depth], H + pl.gemm_off[depth], pl.gemm_cnt[depth]);
   }
   NRG_HIP(hipGetLastError());
