@@ -559,13 +559,23 @@ __global__ __launch_bounds__(NT_YR) void k_col_yred(SyevArgs<T> a, const int2 *i
 // ------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blist, const int *tpre, int nact, int jl) {
-  // block of this workgroup: largest i with tpre[i] <= blockIdx.x
-  int lo = 0, hi = nact - 1;
+  // block of this workgroup: the largest i with tpre[i] <= blockIdx.x, i.e. the count of such entries
+  // less one (tpre ascends from 0). Every wave counts them with ballots over 256 entries per round
+  // trip (a binary search would chain log2(nact) dependent loads before the tile's own)
   const int g = blockIdx.x;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tpre[mid] <= g) lo = mid; else hi = mid - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int cnt = 0;
+  for (int base = 0; base < nact; base += 4 * WAVE) {
+    int tv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = base + u * WAVE + lane;
+      tv[u] = i < nact ? tpre[i] : 0x7fffffff;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) cnt += __popcll(__ballot(tv[u] <= g));
   }
+  const int lo = cnt - 1;
   const int ent = blist[lo], p = ent >> COL_PANEL_SHIFT;
   const BlkDesc bd = a.bd[ent & COL_BLK_MASK];
   const int tl = g - tpre[lo];
@@ -575,7 +585,6 @@ __global__ __launch_bounds__(256) void k_col_symv(SyevArgs<T> a, const int *blis
   const int r0 = jl + 1;                          // trailing rows/cols [r0, m) of the panel
   const int t = m - r0;
   const int nseg = (t + 63) / 64, ntri = nseg * (nseg + 1) / 2;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tl >= ntri) {
     // segment partials of W^T v and V^T v for the next column's k_col_prep (only columns k < jl
     // exist): wave 0/1 take W(:, 0..15 / 16..31), wave 2/3 take V(:, 0..15 / 16..31)
