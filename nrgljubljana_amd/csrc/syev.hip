@@ -251,20 +251,18 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
     __syncthreads();   // sh[1] is reused by the reflector below
     }
   } else {
-  // at most NT rows [jl, m): one row per thread
+  // at most NT rows [jl, m): one row per thread. Every load of the first phase is issued before any
+  // of them is used (a loop over a run-time count of segments, or an LDS store of a loaded value,
+  // between them would wait for each load in turn)
   constexpr int R1 = 1;
-  T v[R1], y[R1], c[R1], aj[R1];
+  constexpr int TS = NT / 64;   // Tt <= TS segments, and at most one segment per wave
+  T v[R1], y[R1], c[R1], aj[R1], ys[R1][TS];
   // 1. loads that depend on nothing computed here
-  if (jl > 0) {
-    const int cl = tid & 63;
-    T s = 0;
-    if ((cl & 31) < jp)
-      for (int sg = tid >> 6; sg < Tt; sg += NW) s += dt[sg * 64 + cl];
-    part[tid >> 6][cl] = s;
-  }
+  T pl = 0, vr = 0, wr = 0;   // pl: this wave's segment partial of column tid & 63
+  if (jl > 0 && ((tid & 63) & 31) < jp && (tid >> 6) < Tt) pl = dt[(tid >> 6) * 64 + (tid & 63)];
   if (tid < NB) {
-    vrow[tid] = (do_col && tid < jl) ? A[(size_t)tid * n + jl] : (T)0;
-    wrow[tid] = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
+    vr = (do_col && tid < jl) ? A[(size_t)tid * n + jl] : (T)0;
+    wr = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
   }
 #pragma unroll
   for (int q = 0; q < R1; q++) {
@@ -274,10 +272,8 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
       if (jl > 0) {
         v[q] = vb[r];
         aj[q] = A[(size_t)jp * n + r];
-        T s = 0;
-#pragma unroll 16
-        for (int sl = 0; sl < Tt; sl++) s += yp[(size_t)sl * n + r];
-        y[q] = s;
+#pragma unroll
+        for (int sl = 0; sl < TS; sl++) ys[q][sl] = sl < Tt ? yp[(size_t)sl * n + r] : (T)0;
       }
       if (do_col) c[q] = A[(size_t)jl * n + r];
     }
@@ -296,6 +292,23 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
       const bool in = r < m && k < jp;
       ap[q][k] = in ? A[(size_t)k * n + r] : (T)0;
       wp[q][k] = in ? W[(size_t)k * n + r] : (T)0;
+    }
+  }
+  // the loaded values into the LDS, and the product's segment partials summed in segment order
+  if (jl > 0) part[tid >> 6][tid & 63] = pl;   // summed from +0 below, as if added to a +0 here
+  if (tid < NB) {
+    vrow[tid] = vr;
+    wrow[tid] = wr;
+  }
+#pragma unroll
+  for (int q = 0; q < R1; q++) {
+    const int r = jl + tid + q * NT;
+    if (r < m && jl > 0) {
+      T s = 0;
+#pragma unroll
+      for (int sl = 0; sl < TS; sl++)
+        if (sl < Tt) s += ys[q][sl];
+      y[q] = s;
     }
   }
   if (jl > 0) {
