@@ -224,22 +224,33 @@ __global__ __launch_bounds__(NT) void k_col_prep(SyevArgs<T> a, const int *blist
     const T *cs = a.cs + bd.offV + i0;
     const T *dpp = a.dpp + bd.offV;
     const int nch = (m - jl + NT_YR - 1) / NT_YR;   // at most NMAX / NT_YR <= NT chunks
-    if (tid == 0) sh[1] = do_col ? A[(size_t)jp * n + jl] : (T)0;   // V(jl, jp)
-    // no closing barrier: the one below, before the reflector, comes first
-    const T a2 = (T)(-0.5) * tv * block_sum<NT, false>(tid < nch ? dpp[tid] : (T)0, red);
+    // every load first, then the block sum that needs the chunks' partials
+    const T vjp = (tid == 0 && do_col) ? A[(size_t)jp * n + jl] : (T)0;   // V(jl, jp)
+    const T dv = tid < nch ? dpp[tid] : (T)0;
+    T wo[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int r = jl + tid + q * NT;
-      v[q] = y[q] = c[q] = aj[q] = 0;
+      v[q] = y[q] = c[q] = aj[q] = wo[q] = 0;
       if (r < m) {
         v[q] = vb[r];
-        y[q] = W[(size_t)jp * n + r] + a2 * v[q];
-        W[(size_t)jp * n + r] = y[q];
-        if (r == jl) sh[0] = y[q];
+        wo[q] = W[(size_t)jp * n + r];
         if (do_col) {
           c[q] = cs[r];
           aj[q] = A[(size_t)jp * n + r];
         }
+      }
+    }
+    if (tid == 0) sh[1] = vjp;
+    // no closing barrier: the one below, before the reflector, comes first
+    const T a2 = (T)(-0.5) * tv * block_sum<NT, false>(dv, red);
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int r = jl + tid + q * NT;
+      if (r < m) {
+        y[q] = wo[q] + a2 * v[q];
+        W[(size_t)jp * n + r] = y[q];
+        if (r == jl) sh[0] = y[q];
       }
     }
     __syncthreads();
@@ -498,37 +509,66 @@ __global__ __launch_bounds__(NT_YR) void k_col_yred(SyevArgs<T> a, const int2 *i
   __shared__ T cwv[2 * NB], vrow[NB], wrow[NB];
   __shared__ T red[NW];
   const int tid = threadIdx.x;
-  {
-    const int cl = tid & 63;
-    T s = 0;
-    if ((cl & 31) < jp) {
-#pragma unroll 8
-      for (int sg = tid >> 6; sg < Tt; sg += NW) s += dt[sg * 64 + cl];
-    }
-    part[tid >> 6][cl] = s;
-  }
+  // The loads of each batch below are issued before any of them is used (loads whose values are
+  // summed or stored to the LDS as they arrive would each wait for the one before); the sums keep
+  // their order: segments sg = wave, wave + NW, ... into ps; slots of full 32-slot chunks alternately
+  // into y and y2, those of the last partial chunk into y, then y += y2.
+  constexpr int DB = 8, YB = 32;
+  const int cl = tid & 63;
+  T ps = 0, vr = 0, wr = 0;
   if (tid < NB) {
-    vrow[tid] = (do_col && tid < jl) ? A[(size_t)tid * n + jl] : (T)0;
-    wrow[tid] = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
+    vr = (do_col && tid < jl) ? A[(size_t)tid * n + jl] : (T)0;
+    wr = (do_col && tid < jp) ? W[(size_t)tid * n + jl] : (T)0;
   }
+  // the first batch of segment partials (segments sgw + u NW) before the product's loads
+  const int sgw = tid >> 6;
+  const bool dcol = (cl & 31) < jp;
+  T d0[DB];
+#pragma unroll
+  for (int u = 0; u < DB; u++) d0[u] = dcol && sgw + u * NW < Tt ? dt[(sgw + u * NW) * 64 + cl] : (T)0;
   const int r = jl + it.y * NT_YR + tid;
   const bool rin = r < m;
   T y = 0, cc = 0, v = 0;
   if (rin) {
-    T y2 = 0;
-    int sl = 0;
-    for (; sl + 32 <= Tt; sl += 32) {   // 32 loads in flight, two accumulators
-#pragma unroll
-      for (int u = 0; u < 32; u += 2) {
-        y += yp[(size_t)(sl + u) * n + r];
-        y2 += yp[(size_t)(sl + u + 1) * n + r];
-      }
-    }
-#pragma unroll 8
-    for (; sl < Tt; sl++) y += yp[(size_t)sl * n + r];
-    y += y2;
     v = vb[r];
     if (do_col) cc = A[(size_t)jl * n + r];
+    T y2 = 0;
+    for (int sl = 0; sl < Tt; sl += YB) {
+      T ys[YB];
+#pragma unroll
+      for (int u = 0; u < YB; u++) ys[u] = sl + u < Tt ? yp[(size_t)(sl + u) * n + r] : (T)0;
+      if (sl + YB <= Tt) {
+#pragma unroll
+        for (int u = 0; u < YB; u += 2) {
+          y += ys[u];
+          y2 += ys[u + 1];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < YB; u++)
+          if (sl + u < Tt) y += ys[u];
+      }
+    }
+    y += y2;
+  }
+  ps = d0[0];   // as 0 + d0[0] up to the sign of a zero, which the sum from +0 over the waves drops
+  if (dcol) {
+#pragma unroll
+    for (int u = 1; u < DB; u++)
+      if (sgw + u * NW < Tt) ps += d0[u];
+    for (int sg0 = sgw + DB * NW; sg0 < Tt; sg0 += DB * NW) {
+      T ds[DB];
+#pragma unroll
+      for (int u = 0; u < DB; u++) ds[u] = sg0 + u * NW < Tt ? dt[(sg0 + u * NW) * 64 + cl] : (T)0;
+#pragma unroll
+      for (int u = 0; u < DB; u++)
+        if (sg0 + u * NW < Tt) ps += ds[u];
+    }
+  }
+  part[tid >> 6][cl] = ps;
+  if (tid < NB) {
+    vrow[tid] = vr;
+    wrow[tid] = wr;
   }
   __syncthreads();
   if (tid < 2 * NB) {
